@@ -1,0 +1,1020 @@
+// fk_engine.hip -- host side of libfkadj: problem set-up, stage-time field tables, the forward /
+// recompute / adjoint sweeps over device-resident checkpoints, batched evaluation and the C ABI
+// declared in include/fkadj.h.
+//
+// Reference host routine: fk_grad_Et_comp, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938 (G).
+// Differences by design: the trajectory never leaves HBM (the reference copies every state to
+// host memory and back, G:1721-1723, G:1871-1872), it is kept as checkpoints every K steps and
+// recomputed one segment at a time, the per-step z.gh partials stay on the device until one
+// fixed-order reduction at the end, and errors are returned instead of exit().
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/fkadj.h"
+#include "fk_internal.hpp"
+
+using namespace fk;
+
+namespace {
+
+thread_local char g_errmsg[256];
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t e_ = (expr);                                                                        \
+        if (e_ != hipSuccess) {                                                                        \
+            snprintf(g_errmsg, sizeof g_errmsg, "%s: %s (%s:%d)", #expr, hipGetErrorString(e_), __FILE__, \
+                     __LINE__);                                                                        \
+            return FKADJ_ERR_DEVICE;                                                                   \
+        }                                                                                              \
+    } while (0)
+
+#define CHK(expr)                      \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != FKADJ_OK) return rc_; \
+    } while (0)
+
+std::atomic<int> g_profiling{0};
+std::mutex g_timing_mu;
+fkadj_timing g_timing{};
+
+// ------------------------------------------------------------------------ spline
+// Second derivatives of the not-a-knot cubic spline through equally or unequally spaced knots
+// (the reference uses Burkardt's spline_cubic_set with ibc = 3, G:1681, G:2223).  The two
+// not-a-knot conditions are folded into the first and last interior equations and the
+// remaining tridiagonal system is solved by elimination.
+bool spline_not_a_knot(int n, const double *t, const double *y, std::vector<double> &ypp)
+{
+    if (n < 4) return false;
+    int m = n - 2;
+    std::vector<double> lo(m), di(m), up(m), rh(m);
+    for (int r = 0; r < m; r++) {
+        int i = r + 1;
+        double h0 = t[i] - t[i - 1], h1 = t[i + 1] - t[i];
+        lo[r] = h0 / 6.0;
+        di[r] = (h0 + h1) / 3.0;
+        up[r] = h1 / 6.0;
+        rh[r] = (y[i + 1] - y[i]) / h1 - (y[i] - y[i - 1]) / h0;
+    }
+    double h0 = t[1] - t[0], h1 = t[2] - t[1];
+    di[0] += lo[0] * (h0 + h1) / h1;
+    up[0] -= lo[0] * h0 / h1;
+    double ha = t[n - 2] - t[n - 3], hb = t[n - 1] - t[n - 2];
+    di[m - 1] += up[m - 1] * (ha + hb) / ha;
+    lo[m - 1] -= up[m - 1] * hb / ha;
+    for (int r = 1; r < m; r++) {
+        double f = lo[r] / di[r - 1];
+        di[r] -= f * up[r - 1];
+        rh[r] -= f * rh[r - 1];
+    }
+    ypp.assign(n, 0.0);
+    ypp[m] = rh[m - 1] / di[m - 1];
+    for (int r = m - 2; r >= 0; r--) ypp[r + 1] = (rh[r] - up[r] * ypp[r + 2]) / di[r];
+    ypp[0] = ((h0 + h1) * ypp[1] - h0 * ypp[2]) / h1;
+    ypp[n - 1] = ((ha + hb) * ypp[n - 2] - hb * ypp[n - 3]) / ha;
+    return true;
+}
+
+double spline_eval(const double *t, const double *y, const std::vector<double> &ypp, int lo, double tv)
+{
+    int hi = lo + 1;
+    double h = t[hi] - t[lo], a = tv - t[lo], b = t[hi] - tv;
+    return (ypp[lo] * b * b * b + ypp[hi] * a * a * a) / (6.0 * h) + (y[lo] / h - ypp[lo] * h / 6.0) * b +
+           (y[hi] / h - ypp[hi] * h / 6.0) * a;
+}
+
+bool all_finite(const double *a, size_t n)
+{
+    for (size_t i = 0; i < n; i++)
+        if (!std::isfinite(a[i])) return false;
+    return true;
+}
+
+}  // namespace
+
+// ============================================================================ instance
+struct fkadj_instance {
+    fkadj_problem p{};
+    int rows = 0, cols = 0, pitch = 0, n = 0, n_num = 0;
+    int nt = 0, K = 0, nck = 0, ncontrols = 0;
+    int prec = FKADJ_F64;
+    size_t esz = 8;        // bytes per state element
+    size_t st_bytes = 0;   // bytes of one 3-plane state
+    Model64 m64{};
+    Model32 m32{};
+    std::vector<int32_t> cell_of;
+    bool state_set = false;
+
+    uint16_t *d_code = nullptr;
+    int32_t *d_cell_of = nullptr;
+    int64_t *d_rowptr = nullptr;
+    int32_t *d_colind = nullptr;
+    double *d_val = nullptr;
+    double *d_gamma = nullptr;
+    char *d_y0 = nullptr, *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
+    double *d_Ef = nullptr, *d_Ea = nullptr;
+    double *d_part = nullptr, *d_partT = nullptr, *d_zgh = nullptr, *d_L0part = nullptr, *d_L0 = nullptr;
+    double *d_packed = nullptr;
+    uint8_t *d_mask = nullptr;
+    hipStream_t stream = nullptr;
+    int nb_plain = 0;     // plain blocks (zgh partial row length of the plain path)
+    int nb_part = 0;      // partial row length actually used by the adjoint sweep
+
+    // fused path
+    int ty_f = 16;
+    FusedSlot slot_h{};
+    struct FusedGroupDev *group = nullptr;   // this instance alone, for fkadj_grad / fkadj_forward
+
+    std::vector<double> Ef, Ea;   // host tables of the last evaluation
+
+    char *state(char *base, int idx) const { return base + (size_t)idx * st_bytes; }
+};
+
+// A set of FP32 instances advanced together: one forward launch per tile height and one adjoint
+// launch per time step cover all members.  A single instance is a group of one.
+struct FusedGroupDev {
+    std::vector<fkadj_instance *> members;
+    FusedSlot *d_slots = nullptr;
+    TileTask *d_f16 = nullptr, *d_f32 = nullptr, *d_adj = nullptr;
+    int n16 = 0, n32 = 0, nadj = 0;
+};
+
+namespace {
+
+template <typename T>
+T *as(char *p) { return reinterpret_cast<T *>(p); }
+
+void fill_models(fkadj_instance *I)
+{
+    const fkadj_model &md = I->p.model;
+    Model64 &a = I->m64;
+    a.u_c = md.u_c; a.u_v = md.u_v; a.tau_d = md.tau_d; a.tau_v1_minus = md.tau_v1_minus;
+    a.tau_v2_minus = md.tau_v2_minus; a.tau_v_plus = md.tau_v_plus; a.tau_0 = md.tau_0; a.tau_r = md.tau_r;
+    a.tau_si = md.tau_si; a.u_csi = md.u_csi; a.k_fk = md.k_fk; a.tau_w_minus = md.tau_w_minus;
+    a.tau_w_plus = md.tau_w_plus; a.ksm = md.ksm;
+    a.lc[0] = md.Du / (md.h * md.h);
+    a.lc[1] = md.Dv / (md.h * md.h);
+    a.lc[2] = md.Dw / (md.h * md.h);
+    a.ghc = md.Du * 2 / md.h;
+
+    Model32 &b = I->m32;
+    const double L2E = 1.4426950408889634;
+    b.kq = (float)(-2.0 * md.ksm * L2E);
+    b.bq = (float)(2.0 * md.ksm * md.u_v * L2E);
+    b.cpq = (float)std::exp(-2.0 * md.ksm * (md.u_c - md.u_v));
+    b.ks = (float)(-2.0 * md.k_fk * L2E);
+    b.bs = (float)(2.0 * md.k_fk * md.u_csi * L2E);
+    b.two_ksm = (float)(2.0 * md.ksm);
+    b.two_k = (float)(2.0 * md.k_fk);
+    b.u_c = (float)md.u_c;
+    b.one_plus_uc = (float)(1.0 + md.u_c);
+    b.i_tau_d = (float)(1.0 / md.tau_d);
+    b.i_tau_0 = (float)(1.0 / md.tau_0);
+    b.i_tau_r = (float)(1.0 / md.tau_r);
+    b.i_tau_si = (float)(1.0 / md.tau_si);
+    b.i_tau_vp = (float)(1.0 / md.tau_v_plus);
+    b.i_tau_wm = (float)(1.0 / md.tau_w_minus);
+    b.i_tau_wp = (float)(1.0 / md.tau_w_plus);
+    b.tau_v2m = (float)md.tau_v2_minus;
+    b.dtv = (float)(md.tau_v1_minus - md.tau_v2_minus);
+    for (int v = 0; v < 3; v++) b.lc[v] = (float)a.lc[v];
+    b.ghc = (float)a.ghc;
+}
+
+// Fused tile height per grid: 32-row forward tiles halve the halo recomputation but give half
+// as many workgroups; small grids keep 16 rows so that one instance still fills the GPU.
+int autotune_tile_rows(int rows, int pitch)
+{
+    long tiles32 = (long)(pitch / FUSED_TX) * ((rows + 31) / 32);
+    return tiles32 >= 512 ? 32 : 16;
+}
+
+// XCD-aware order: the GPU deals workgroups round-robin over its 8 XCDs, each with its own L2.
+// Block b runs on XCD b % 8; giving XCD x a contiguous run of tiles keeps neighbouring tiles
+// (which share halo cache lines) in the same L2.  The map is a bijection for any count.
+std::vector<TileTask> xcd_order(const std::vector<TileTask> &in)
+{
+    const size_t X = 8, N = in.size();
+    std::vector<size_t> start(X + 1, 0);
+    for (size_t x = 0; x < X; x++) start[x + 1] = start[x] + (N > x ? (N - x + X - 1) / X : 0);
+    std::vector<TileTask> out(N);
+    for (size_t b = 0; b < N; b++) out[b] = in[start[b % X] + b / X];
+    return out;
+}
+
+std::vector<TileTask> make_tiles(int slot, int rows, int pitch, int ty)
+{
+    std::vector<TileTask> v;
+    int idx = 0;
+    for (int y0 = 0; y0 < rows; y0 += ty)
+        for (int x0 = 0; x0 < pitch; x0 += FUSED_TX) v.push_back(TileTask{slot, x0, y0, idx++});
+    return v;
+}
+
+int validate_and_codes(const fkadj_problem *p, int pitch, std::vector<uint16_t> &code, std::vector<int32_t> &cell_of,
+                       int &n_num)
+{
+    const int rows = p->nrows, cols = p->ncols;
+    n_num = 0;
+    for (int k = 0; k < rows * cols; k++)
+        if (p->active_index[k] > 0) n_num++;
+    if (n_num == 0) return FKADJ_ERR_GEOMETRY;
+    cell_of.assign(n_num, -1);
+    for (int i = 0; i < rows; i++)
+        for (int j = 0; j < cols; j++) {
+            int a = p->active_index[i * cols + j];
+            if (a < 0 || a > n_num) return FKADJ_ERR_GEOMETRY;
+            if (a > 0) {
+                if (cell_of[a - 1] != -1) return FKADJ_ERR_GEOMETRY;
+                cell_of[a - 1] = i * pitch + j;
+            }
+        }
+    auto act = [&](int i, int j) -> bool {
+        return i >= 0 && i < rows && j >= 0 && j < cols && p->active_index[i * cols + j] > 0;
+    };
+    code.assign((size_t)rows * pitch, 0);
+    for (int i = 0; i < rows; i++)
+        for (int j = 0; j < cols; j++) {
+            int a = p->active_index[i * cols + j];
+            if (a <= 0) continue;
+            uint32_t c = C_ACTIVE;
+            bool t = act(i - 1, j), b = act(i + 1, j), l = act(i, j - 1), r = act(i, j + 1);
+            if ((!t && !b) || (!l && !r)) return FKADJ_ERR_GEOMETRY;   // mirror rule undefined (NaN in G)
+            if (!t) c |= C_TOP_MISS;
+            if (!b) c |= C_BOT_MISS;
+            if (!l) c |= C_LEFT_MISS;
+            if (!r) c |= C_RIGHT_MISS;
+            auto w = [&](int di, int dj) -> uint32_t {
+                if (!act(i + di, j + dj)) return 0u;
+                return act(i + 2 * di, j + 2 * dj) ? 1u : 2u;
+            };
+            c |= w(-1, 0) << C_WT_SHIFT;
+            c |= w(1, 0) << C_WB_SHIFT;
+            c |= w(0, -1) << C_WL_SHIFT;
+            c |= w(0, 1) << C_WR_SHIFT;
+            int gh;
+            if (p->gh_num) {
+                double g = p->gh_num[a - 1];
+                if (g == 1.0) gh = 1;
+                else if (g == -1.0) gh = -1;
+                else if (g == 0.0) gh = 0;
+                else return FKADJ_ERR_ARG;
+            } else {
+                gh = (int)!b - (int)!t;   // [i+1 missing] - [i-1 missing]
+            }
+            c |= (uint32_t)(gh + 1) << C_GH_SHIFT;
+            code[(size_t)i * pitch + j] = (uint16_t)c;
+        }
+    return FKADJ_OK;
+}
+
+template <typename T>
+PlainCtx<T, typename std::conditional<std::is_same<T, double>::value, Model64, Model32>::type>
+plain_ctx(fkadj_instance *I)
+{
+    PlainCtx<T, typename std::conditional<std::is_same<T, double>::value, Model64, Model32>::type> c;
+    c.g = GridArgs{I->rows, I->pitch, I->n, I->d_code};
+    if constexpr (std::is_same<T, double>::value) c.m = I->m64; else c.m = I->m32;
+    c.dt = I->p.dt;
+    c.Ef = I->d_Ef;
+    c.Ea = I->d_Ea;
+    T *w = as<T>(I->d_work);
+    size_t s = (size_t)3 * I->n;
+    c.w0 = w; c.w1 = w + s; c.w2 = w + 2 * s; c.w3 = w + 3 * s; c.w4 = w + 4 * s; c.w5 = w + 5 * s;
+    c.stream = I->stream;
+    return c;
+}
+
+// Location of forward state `step` during the forward sweep (checkpoint or ring buffer).
+char *fwd_loc(fkadj_instance *I, int step)
+{
+    if (step % I->K == 0 || step == I->nt) return I->state(I->d_ck, (step + I->K - 1) / I->K);
+    return I->state(I->d_ring, step & 1);
+}
+
+// Stage-time field tables (see fkadj.h for the two pulse modes).
+int build_tables(fkadj_instance *I, const double *E)
+{
+    const int nt = I->nt;
+    I->Ef.assign((size_t)3 * nt, 0.0);
+    I->Ea.assign((size_t)3 * nt, 0.0);
+    if (I->p.pulse_mode == FKADJ_PULSE_KNOTS) {
+        std::vector<double> t(nt + 1), ypp;
+        for (int i = 0; i <= nt; i++) t[i] = i * I->p.dt;
+        if (!spline_not_a_knot(nt + 1, t.data(), E, ypp)) return FKADJ_ERR_ARG;
+        for (int i = 0; i < nt; i++) {
+            I->Ef[3 * i] = E[i];
+            I->Ef[3 * i + 1] = spline_eval(t.data(), E, ypp, i, (i + 0.5) * I->p.dt);        // G:1692-1694
+            I->Ef[3 * i + 2] = E[i + 1];
+        }
+        for (int i = 1; i <= nt; i++) {
+            I->Ea[3 * (i - 1)] = E[i - 1];
+            I->Ea[3 * (i - 1) + 1] = E[i];
+            I->Ea[3 * (i - 1) + 2] = spline_eval(t.data(), E, ypp, i - 1, (i - 1 + 1.0 / 3.0) * I->p.dt);  // G:1877
+        }
+    } else {
+        const int P = nt / I->p.n_amp;
+        for (int i = 0; i < nt; i++) {
+            double a = E[i / P];
+            I->Ef[3 * i] = I->Ef[3 * i + 1] = I->Ef[3 * i + 2] = a;
+            I->Ea[3 * i] = I->Ea[3 * i + 1] = I->Ea[3 * i + 2] = a;
+        }
+    }
+    HIPCHK(hipMemcpyAsync(I->d_Ef, I->Ef.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, I->stream));
+    HIPCHK(hipMemcpyAsync(I->d_Ea, I->Ea.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, I->stream));
+    return FKADJ_OK;
+}
+
+void assemble(const fkadj_instance *I, const double *E, const double *zgh, double L0, double *grad, double *Lc,
+              double *L0o, double *LEo)
+{
+    const int nt = I->nt;
+    const double dt = I->p.dt, alpha = I->p.alpha;
+    double LE;
+    if (I->p.pulse_mode == FKADJ_PULSE_KNOTS) {
+        double e2 = 0.0;
+        for (int i = 0; i <= nt; i++) e2 += E[i] * E[i];
+        LE = dt * e2;                                                 // G:1803
+        if (grad)
+            for (int i = 0; i <= nt; i++) grad[i] = alpha * E[i] - zgh[i];   // G:1924-1926
+    } else {
+        const int P = nt / I->p.n_amp;
+        const double Tp = P * dt;
+        double e2 = 0.0;
+        for (int k = 0; k < I->p.n_amp; k++) e2 += E[k] * E[k];
+        LE = Tp * e2;
+        if (grad)
+            for (int k = 0; k < I->p.n_amp; k++) {
+                double q = 0.0;
+                for (int s = k * P; s < (k + 1) * P; s++) q += 0.5 * (zgh[s] + zgh[s + 1]);
+                grad[k] = alpha * E[k] * Tp - dt * q;
+            }
+    }
+    if (L0o) *L0o = L0;
+    if (LEo) *LEo = LE;
+    if (Lc) *Lc = 0.5 * L0 + 0.5 * alpha * LE;   // G:1804
+}
+
+struct EventPool {
+    std::vector<hipEvent_t> ev;
+    size_t used = 0;
+    hipEvent_t get()
+    {
+        if (used == ev.size()) {
+            hipEvent_t e;
+            if (hipEventCreate(&e) != hipSuccess) return nullptr;
+            ev.push_back(e);
+        }
+        return ev[used++];
+    }
+    void reset() { used = 0; }
+    ~EventPool()
+    {
+        for (auto e : ev) (void)hipEventDestroy(e);
+    }
+};
+
+constexpr int kSampleEvery = 8;   // profile one launch in 8
+
+struct Sampler {
+    bool on;
+    EventPool pool;
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd, adj;
+    int64_t nf = 0, na = 0;
+    explicit Sampler(bool o) : on(o) {}
+    template <typename F>
+    void fwd_launch(hipStream_t s, F f)
+    {
+        if (on && (nf % kSampleEvery) == 0) {
+            hipEvent_t a = pool.get(), b = pool.get();
+            (void)hipEventRecord(a, s);
+            f();
+            (void)hipEventRecord(b, s);
+            fwd.push_back({a, b});
+        } else {
+            f();
+        }
+        nf++;
+    }
+    template <typename F>
+    void adj_launch(hipStream_t s, F f)
+    {
+        if (on && (na % kSampleEvery) == 0) {
+            hipEvent_t a = pool.get(), b = pool.get();
+            (void)hipEventRecord(a, s);
+            f();
+            (void)hipEventRecord(b, s);
+            adj.push_back({a, b});
+        } else {
+            f();
+        }
+        na++;
+    }
+    static double mean(const std::vector<std::pair<hipEvent_t, hipEvent_t>> &v)
+    {
+        if (v.empty()) return 0.0;
+        double s = 0.0;
+        for (auto &p : v) {
+            float ms = 0.f;
+            (void)hipEventElapsedTime(&ms, p.first, p.second);
+            s += ms;
+        }
+        return s / v.size();
+    }
+};
+
+// ------------------------------------------------------------------------ one evaluation
+// Forward sweep with checkpoints, objective and terminal adjoint, then per segment (last to
+// first) recompute + adjoint steps.  All launches go to one stream.
+template <typename T>
+void terminal(fkadj_instance *I, hipStream_t s)
+{
+    auto pc = plain_ctx<T>(I);
+    pc.stream = s;
+    T *yT = as<T>(I->state(I->d_ck, I->nck - 1));
+    T *z0 = as<T>(I->state(I->d_z, 0));
+    objective_launch<T>(I->n_num, I->d_rowptr, I->d_colind, I->d_val, I->p.s_diag, I->d_cell_of, I->n, yT,
+                        I->d_gamma, z0, I->d_L0part, s);
+    sum_rows_launch(I->d_L0part, objective_blocks(I->n_num), 1, I->d_L0, s);
+    plain_zgh(pc, z0, I->d_partT);
+    sum_rows_launch(I->d_partT, I->nb_plain, 1, I->d_zgh + I->nt, s);
+}
+
+int run_plain(fkadj_instance *I, Sampler &smp)
+{
+    using T = double;
+    const int nt = I->nt, K = I->K;
+    hipStream_t s = I->stream;
+    auto pc = plain_ctx<T>(I);
+    HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    for (int i = 0; i < nt; i++) {
+        T *yin = as<T>(fwd_loc(I, i)), *yout = as<T>(fwd_loc(I, i + 1));
+        smp.fwd_launch(s, [&] { plain_forward_step(pc, yin, yout, i); });
+    }
+    HIPCHK(hipGetLastError());
+    terminal<T>(I, s);
+    const int nseg = (nt + K - 1) / K;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int a = sg * K, b = std::min(a + K, nt);
+        for (int j = 0; j + 1 < b - a; j++) {
+            T *yin = as<T>(j == 0 ? I->state(I->d_ck, sg) : I->state(I->d_seg, j));
+            T *yout = as<T>(I->state(I->d_seg, j + 1));
+            smp.fwd_launch(s, [&] { plain_forward_step(pc, yin, yout, a + j); });
+        }
+        for (int i = b; i > a; i--) {
+            T *yA = as<T>(i - 1 == a ? I->state(I->d_ck, sg) : I->state(I->d_seg, i - 1 - a));
+            T *yB = as<T>(i == b ? I->state(I->d_ck, sg + 1) : I->state(I->d_seg, i - a));
+            T *zin = as<T>(I->state(I->d_z, (nt - i) & 1));
+            T *zout = as<T>(I->state(I->d_z, (nt - i + 1) & 1));
+            double *prow = I->d_part + (size_t)(i - 1) * I->nb_part;
+            smp.adj_launch(s, [&] { plain_adjoint_step(pc, yA, yB, zin, zout, i - 1, prow); });
+        }
+        HIPCHK(hipGetLastError());
+    }
+    sum_rows_launch(I->d_part, I->nb_part, nt, I->d_zgh, s);
+    HIPCHK(hipGetLastError());
+    return FKADJ_OK;
+}
+
+void group_free(FusedGroupDev *g)
+{
+    if (!g) return;
+    if (g->d_slots) (void)hipFree(g->d_slots);
+    if (g->d_f16) (void)hipFree(g->d_f16);
+    if (g->d_f32) (void)hipFree(g->d_f32);
+    if (g->d_adj) (void)hipFree(g->d_adj);
+    delete g;
+}
+
+int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, FusedGroupDev **out)
+{
+    auto *g = new (std::nothrow) FusedGroupDev();
+    if (!g) return FKADJ_ERR_NOMEM;
+    g->members = members;
+    std::vector<FusedSlot> slots;
+    std::vector<TileTask> f16, f32, adj;
+    for (size_t k = 0; k < members.size(); k++) {
+        fkadj_instance *I = members[k];
+        slots.push_back(I->slot_h);
+        auto ft = make_tiles((int)k, I->rows, I->pitch, I->ty_f);
+        (I->ty_f == 32 ? f32 : f16).insert((I->ty_f == 32 ? f32 : f16).end(), ft.begin(), ft.end());
+        auto at = make_tiles((int)k, I->rows, I->pitch, 16);
+        adj.insert(adj.end(), at.begin(), at.end());
+    }
+    f16 = xcd_order(f16);
+    f32 = xcd_order(f32);
+    adj = xcd_order(adj);
+    g->n16 = (int)f16.size();
+    g->n32 = (int)f32.size();
+    g->nadj = (int)adj.size();
+    bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
+              hipMalloc((void **)&g->d_adj, sizeof(TileTask) * adj.size()) == hipSuccess &&
+              (f16.empty() || hipMalloc((void **)&g->d_f16, sizeof(TileTask) * f16.size()) == hipSuccess) &&
+              (f32.empty() || hipMalloc((void **)&g->d_f32, sizeof(TileTask) * f32.size()) == hipSuccess);
+    if (!ok) { group_free(g); return FKADJ_ERR_NOMEM; }
+    ok = hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+         hipMemcpyAsync(g->d_adj, adj.data(), sizeof(TileTask) * adj.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
+         (f16.empty() || hipMemcpyAsync(g->d_f16, f16.data(), sizeof(TileTask) * f16.size(), hipMemcpyHostToDevice, s) == hipSuccess) &&
+         (f32.empty() || hipMemcpyAsync(g->d_f32, f32.data(), sizeof(TileTask) * f32.size(), hipMemcpyHostToDevice, s) == hipSuccess) &&
+         hipStreamSynchronize(s) == hipSuccess;
+    if (!ok) { group_free(g); return FKADJ_ERR_DEVICE; }
+    *out = g;
+    return FKADJ_OK;
+}
+
+void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
+{
+    if (g->n16) fused_forward_launch(16, g->d_slots, g->d_f16, g->n16, mode, step, seg_a, s);
+    if (g->n32) fused_forward_launch(32, g->d_slots, g->d_f32, g->n32, mode, step, seg_a, s);
+}
+
+int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
+{
+    fkadj_instance *I0 = g->members[0];
+    const int nt = I0->nt, K = I0->K;
+    for (auto *I : g->members)
+        HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
+    HIPCHK(hipGetLastError());
+    for (auto *I : g->members) terminal<float>(I, s);
+    const int nseg = (nt + K - 1) / K;
+    for (int sg = nseg - 1; sg >= 0; sg--) {
+        const int a = sg * K, b = std::min(a + K, nt);
+        for (int j = 0; j + 1 < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
+        for (int i = b; i > a; i--)
+            smp.adj_launch(s, [&] { fused_adjoint_launch(16, g->d_slots, g->d_adj, g->nadj, i, a, s); });
+        HIPCHK(hipGetLastError());
+    }
+    for (auto *I : g->members) sum_rows_launch(I->d_part, I->nb_part, nt, I->d_zgh, s);
+    HIPCHK(hipGetLastError());
+    return FKADJ_OK;
+}
+
+void record_timing(Sampler &smp, hipEvent_t e0, hipEvent_t e1, int64_t cell_steps)
+{
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    g_timing.eval_ms = ms;
+    g_timing.fwd_kernel_ms = Sampler::mean(smp.fwd);
+    g_timing.adj_kernel_ms = Sampler::mean(smp.adj);
+    g_timing.fwd_launches = smp.nf;
+    g_timing.adj_launches = smp.na;
+    g_timing.cell_steps = cell_steps;
+}
+
+int fetch_and_assemble(fkadj_instance *I, hipStream_t s, const double *E, double *grad, double *Lc, double *L0,
+                       double *LE)
+{
+    std::vector<double> zgh(I->nt + 1);
+    double L0v = 0.0;
+    HIPCHK(hipMemcpyAsync(zgh.data(), I->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&L0v, I->d_L0, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (!all_finite(zgh.data(), zgh.size()) || !std::isfinite(L0v)) return FKADJ_ERR_NONFINITE;
+    assemble(I, E, zgh.data(), L0v, grad, Lc, L0, LE);
+    return FKADJ_OK;
+}
+
+int eval_one(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
+{
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    CHK(build_tables(I, E));
+    Sampler smp(g_profiling.load() != 0);
+    hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
+    HIPCHK(hipEventRecord(e0, I->stream));
+    int rc = (I->prec == FKADJ_F64) ? run_plain(I, smp) : run_fused(I->group, I->stream, smp);
+    if (rc) return rc;
+    HIPCHK(hipEventRecord(e1, I->stream));
+    CHK(fetch_and_assemble(I, I->stream, E, grad, Lc, L0, LE));
+    record_timing(smp, e0, e1, (int64_t)I->rows * I->pitch * I->nt);
+    return FKADJ_OK;
+}
+
+template <typename T>
+void dfree(T *&p)
+{
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+
+}  // namespace
+
+// ============================================================================ C ABI
+extern "C" {
+
+const char *fkadj_strerror(int code)
+{
+    switch (code) {
+    case FKADJ_OK: return "ok";
+    case FKADJ_ERR_ARG: return "invalid argument";
+    case FKADJ_ERR_GEOMETRY: return "invalid grid geometry (both axial neighbours of an active cell missing, "
+                                    "bad active_index, or S size mismatch)";
+    case FKADJ_ERR_NOMEM: return "device memory exhausted or over budget";
+    case FKADJ_ERR_DEVICE: return g_errmsg[0] ? g_errmsg : "HIP runtime error";
+    case FKADJ_ERR_NONFINITE: return "non-finite input or result";
+    case FKADJ_ERR_STATE: return "call out of order (no state set)";
+    case FKADJ_ERR_CANCELLED: return "request cancelled";
+    case FKADJ_ERR_BUSY: return "result not ready";
+    case FKADJ_ERR_NOTFOUND: return "unknown request";
+    default: return "unknown error";
+    }
+}
+
+void fkadj_model_reference(fkadj_model *m)
+{
+    // src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24
+    m->u_c = 0.13; m->u_v = 0.04; m->tau_d = 0.388; m->tau_v1_minus = 7.6; m->tau_v2_minus = 8.8;
+    m->tau_v_plus = 3.324; m->tau_0 = 8.2; m->tau_r = 33.264; m->tau_si = 29.0; m->u_csi = 0.54; m->k_fk = 15.0;
+    m->tau_w_minus = 68.0; m->tau_w_plus = 399.99999999999994; m->Du = 1.0e-3; m->Dv = 1.0e-5; m->Dw = 1.0e-5;
+    m->ksm = 100.0; m->h = 0.035;
+}
+
+int fkadj_param_string(const fkadj_model *m, char *buf, size_t cap)
+{
+    if (!m) return FKADJ_ERR_ARG;
+    char tmp[1024];
+    int len = snprintf(tmp, sizeof tmp,
+                       "// Fenton-Karma parameters (libfkadj)\nu_c %.17g\nu_v %.17g\ntau_d %.17g\ntau_v1_minus %.17g\n"
+                       "tau_v2_minus %.17g\ntau_v_plus %.17g\ntau_0 %.17g\ntau_r %.17g\ntau_si %.17g\nu_csi %.17g\n"
+                       "k_fk %.17g\ntau_w_minus %.17g\ntau_w_plus %.17g\nDu %.17g\nDv %.17g\nDw %.17g\nksm %.17g\n"
+                       "h %.17g\nnvar 3\n",
+                       m->u_c, m->u_v, m->tau_d, m->tau_v1_minus, m->tau_v2_minus, m->tau_v_plus, m->tau_0, m->tau_r,
+                       m->tau_si, m->u_csi, m->k_fk, m->tau_w_minus, m->tau_w_plus, m->Du, m->Dv, m->Dw, m->ksm, m->h);
+    if (buf && cap > 0) {
+        size_t c = std::min((size_t)len, cap - 1);
+        memcpy(buf, tmp, c);
+        buf[c] = '\0';
+    }
+    return len;
+}
+
+int64_t fkadj_footprint_bytes(const fkadj_problem *p)
+{
+    if (!p || p->nrows < 1 || p->ncols < 1 || p->nt < 1) return -1;
+    const int pitch = (p->ncols + 63) / 64 * 64;
+    const int64_t n = (int64_t)p->nrows * pitch;
+    const int K = p->ckpt > 0 ? p->ckpt : 50;
+    const int64_t esz = p->precision == FKADJ_F32 ? 4 : 8;
+    const int64_t st = 3 * n * esz;
+    const int64_t nck = (p->nt + K - 1) / K + 1;
+    int64_t states = 1 + nck + (K + 1) + 2 + 2 + (p->precision == FKADJ_F64 ? 6 : 0);
+    int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : (pitch / 64) * ((p->nrows + 15) / 16);
+    int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
+                    8 * 3 * n;
+    return states * st + extra;
+}
+
+int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
+{
+    if (!p || !out || !p->active_index) return FKADJ_ERR_ARG;
+    *out = nullptr;
+    if (p->nrows < 4 || p->ncols < 4 || p->nt < 1 || !(p->dt > 0) || !std::isfinite(p->dt)) return FKADJ_ERR_ARG;
+    if (p->precision != FKADJ_F64 && p->precision != FKADJ_F32) return FKADJ_ERR_ARG;
+    if (p->pulse_mode == FKADJ_PULSE_KNOTS) {
+        if (p->nt + 1 < 4) return FKADJ_ERR_ARG;
+    } else if (p->pulse_mode == FKADJ_PULSE_PIECEWISE) {
+        if (p->n_amp < 1 || p->nt % p->n_amp != 0) return FKADJ_ERR_ARG;
+    } else {
+        return FKADJ_ERR_ARG;
+    }
+    auto *I = new (std::nothrow) fkadj_instance();
+    if (!I) return FKADJ_ERR_NOMEM;
+    I->p = *p;
+    I->rows = p->nrows;
+    I->cols = p->ncols;
+    I->pitch = (p->ncols + 63) / 64 * 64;
+    I->n = I->rows * I->pitch;
+    I->nt = p->nt;
+    I->K = p->ckpt > 0 ? p->ckpt : 50;
+    I->nck = (I->nt + I->K - 1) / I->K + 1;
+    I->prec = p->precision;
+    I->esz = I->prec == FKADJ_F32 ? 4 : 8;
+    I->st_bytes = (size_t)3 * I->n * I->esz;
+    I->ncontrols = p->pulse_mode == FKADJ_PULSE_KNOTS ? p->nt + 1 : p->n_amp;
+    fill_models(I);
+
+    std::vector<uint16_t> code;
+    int rc = validate_and_codes(p, I->pitch, code, I->cell_of, I->n_num);
+    if (rc) { delete I; return rc; }
+    I->p.active_index = nullptr;
+    I->p.gh_num = nullptr;
+
+    // objective matrix: CSC over packed indices -> CSR (rows keep increasing column order)
+    std::vector<int64_t> rowptr;
+    std::vector<int32_t> colind;
+    std::vector<double> val;
+    if (p->S_colptr) {
+        if (!p->S_rowind || !p->S_val) { delete I; return FKADJ_ERR_ARG; }
+        const int nn = I->n_num;
+        int64_t nnz = p->S_colptr[nn];
+        if (p->S_colptr[0] != 0 || nnz < 0) { delete I; return FKADJ_ERR_GEOMETRY; }
+        rowptr.assign(nn + 1, 0);
+        for (int c = 0; c < nn; c++) {
+            if (p->S_colptr[c + 1] < p->S_colptr[c]) { delete I; return FKADJ_ERR_GEOMETRY; }
+            for (int64_t q = p->S_colptr[c]; q < p->S_colptr[c + 1]; q++) {
+                int64_t r = p->S_rowind[q];
+                if (r < 0 || r >= nn) { delete I; return FKADJ_ERR_GEOMETRY; }
+                rowptr[r + 1]++;
+            }
+        }
+        for (int r = 0; r < nn; r++) rowptr[r + 1] += rowptr[r];
+        colind.resize(nnz);
+        val.resize(nnz);
+        std::vector<int64_t> fillp(rowptr.begin(), rowptr.end() - 1);
+        for (int c = 0; c < nn; c++)
+            for (int64_t q = p->S_colptr[c]; q < p->S_colptr[c + 1]; q++) {
+                int64_t r = p->S_rowind[q];
+                colind[fillp[r]] = c;
+                val[fillp[r]++] = p->S_val[q];
+            }
+        if (!all_finite(val.data(), val.size())) { delete I; return FKADJ_ERR_NONFINITE; }
+    }
+    I->p.S_colptr = nullptr;
+    I->p.S_rowind = nullptr;
+    I->p.S_val = nullptr;
+
+    auto fail = [&](int code_) {
+        fkadj_destroy(I);
+        return code_;
+    };
+#define DMALLOC(ptr, bytes)                                                  \
+    do {                                                                     \
+        if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) return fail(FKADJ_ERR_NOMEM); \
+    } while (0)
+
+    if (hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking) != hipSuccess) {
+        snprintf(g_errmsg, sizeof g_errmsg, "hipStreamCreate failed (no GPU?)");
+        delete I;
+        return FKADJ_ERR_DEVICE;
+    }
+    const size_t st = I->st_bytes;
+    const int nt = I->nt;
+    I->nb_plain = (I->pitch / PLAIN_BX) * ((I->rows + PLAIN_BY - 1) / PLAIN_BY);
+    if (I->prec == FKADJ_F32) {
+        I->ty_f = p->tile_rows == 32 || p->tile_rows == 16 ? p->tile_rows : autotune_tile_rows(I->rows, I->pitch);
+        I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + 15) / 16);   // adjoint tiles (16 rows)
+    } else {
+        I->nb_part = I->nb_plain;
+    }
+    DMALLOC(I->d_code, sizeof(uint16_t) * I->n);
+    DMALLOC(I->d_cell_of, sizeof(int32_t) * I->n_num);
+    DMALLOC(I->d_gamma, sizeof(double) * 3);
+    DMALLOC(I->d_y0, st);
+    DMALLOC(I->d_ck, st * I->nck);
+    DMALLOC(I->d_seg, st * (I->K + 1));
+    DMALLOC(I->d_ring, st * 2);
+    DMALLOC(I->d_z, st * 2);
+    if (I->prec == FKADJ_F64) DMALLOC(I->d_work, st * 6);
+    DMALLOC(I->d_Ef, sizeof(double) * 3 * nt);
+    DMALLOC(I->d_Ea, sizeof(double) * 3 * nt);
+    DMALLOC(I->d_part, sizeof(double) * (size_t)nt * I->nb_part);
+    DMALLOC(I->d_partT, sizeof(double) * I->nb_plain);
+    DMALLOC(I->d_zgh, sizeof(double) * (nt + 1));
+    DMALLOC(I->d_L0part, sizeof(double) * objective_blocks(I->n_num));
+    DMALLOC(I->d_L0, sizeof(double));
+    DMALLOC(I->d_packed, sizeof(double) * 3 * I->n_num);
+    DMALLOC(I->d_mask, I->n);
+    if (!rowptr.empty()) {
+        DMALLOC(I->d_rowptr, sizeof(int64_t) * rowptr.size());
+        DMALLOC(I->d_colind, sizeof(int32_t) * std::max<size_t>(1, colind.size()));
+        DMALLOC(I->d_val, sizeof(double) * std::max<size_t>(1, val.size()));
+    }
+    // zero every state buffer once: hole cells stay zero
+    if (hipMemsetAsync(I->d_y0, 0, st, I->stream) != hipSuccess || hipMemsetAsync(I->d_ck, 0, st * I->nck, I->stream) != hipSuccess ||
+        hipMemsetAsync(I->d_seg, 0, st * (I->K + 1), I->stream) != hipSuccess ||
+        hipMemsetAsync(I->d_ring, 0, st * 2, I->stream) != hipSuccess || hipMemsetAsync(I->d_z, 0, st * 2, I->stream) != hipSuccess)
+        return fail(FKADJ_ERR_DEVICE);
+    if (I->d_work && hipMemsetAsync(I->d_work, 0, st * 6, I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
+    bool ok = hipMemcpyAsync(I->d_code, code.data(), sizeof(uint16_t) * I->n, hipMemcpyHostToDevice, I->stream) == hipSuccess &&
+              hipMemcpyAsync(I->d_cell_of, I->cell_of.data(), sizeof(int32_t) * I->n_num, hipMemcpyHostToDevice, I->stream) == hipSuccess &&
+              hipMemcpyAsync(I->d_gamma, p->gamma, sizeof(double) * 3, hipMemcpyHostToDevice, I->stream) == hipSuccess;
+    if (ok && !rowptr.empty())
+        ok = hipMemcpyAsync(I->d_rowptr, rowptr.data(), sizeof(int64_t) * rowptr.size(), hipMemcpyHostToDevice, I->stream) == hipSuccess &&
+             (colind.empty() || (hipMemcpyAsync(I->d_colind, colind.data(), sizeof(int32_t) * colind.size(), hipMemcpyHostToDevice, I->stream) == hipSuccess &&
+                                 hipMemcpyAsync(I->d_val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, I->stream) == hipSuccess));
+    if (!ok) return fail(FKADJ_ERR_DEVICE);
+    if (I->prec == FKADJ_F32) {
+        FusedSlot &S = I->slot_h;
+        S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
+        S.nt = nt; S.K = I->K; S.nck = I->nck;
+        S.ntiles = I->nb_part;
+        S.code = I->d_code;
+        S.ck = as<float>(I->d_ck); S.seg = as<float>(I->d_seg); S.ring = as<float>(I->d_ring); S.zb = as<float>(I->d_z);
+        S.Ef = I->d_Ef; S.Ea = I->d_Ea; S.partial = I->d_part;
+        S.m = I->m32; S.dt = (float)p->dt;
+        std::vector<fkadj_instance *> self{I};
+        int grc = group_build(self, I->stream, &I->group);
+        if (grc) return fail(grc);
+    }
+#undef DMALLOC
+    if (hipStreamSynchronize(I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
+    *out = I;
+    return FKADJ_OK;
+}
+
+int fkadj_destroy(fkadj_instance *I)
+{
+    if (!I) return FKADJ_ERR_ARG;
+    if (I->stream) (void)hipStreamSynchronize(I->stream);
+    dfree(I->d_code); dfree(I->d_cell_of); dfree(I->d_rowptr); dfree(I->d_colind); dfree(I->d_val); dfree(I->d_gamma);
+    dfree(I->d_y0); dfree(I->d_ck); dfree(I->d_seg); dfree(I->d_ring); dfree(I->d_z); dfree(I->d_work);
+    dfree(I->d_Ef); dfree(I->d_Ea); dfree(I->d_part); dfree(I->d_partT); dfree(I->d_zgh); dfree(I->d_L0part);
+    dfree(I->d_L0); dfree(I->d_packed); dfree(I->d_mask); group_free(I->group); I->group = nullptr;
+    if (I->stream) (void)hipStreamDestroy(I->stream);
+    delete I;
+    return FKADJ_OK;
+}
+
+int fkadj_n_active(const fkadj_instance *I) { return I ? I->n_num : FKADJ_ERR_ARG; }
+int fkadj_n_controls(const fkadj_instance *I) { return I ? I->ncontrols : FKADJ_ERR_ARG; }
+
+int fkadj_set_state(fkadj_instance *I, const double *y)
+{
+    if (!I || !y) return FKADJ_ERR_ARG;
+    if (!all_finite(y, (size_t)3 * I->n_num)) return FKADJ_ERR_NONFINITE;
+    HIPCHK(hipMemcpyAsync(I->d_packed, y, sizeof(double) * 3 * I->n_num, hipMemcpyHostToDevice, I->stream));
+    if (I->prec == FKADJ_F64)
+        unpack_launch<double>(I->d_packed, I->d_cell_of, I->n_num, I->n, as<double>(I->d_y0), I->stream);
+    else
+        unpack_launch<float>(I->d_packed, I->d_cell_of, I->n_num, I->n, as<float>(I->d_y0), I->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(I->stream));
+    I->state_set = true;
+    return FKADJ_OK;
+}
+
+static int pack_state(fkadj_instance *I, const char *planes, double *y)
+{
+    if (I->prec == FKADJ_F64)
+        pack_launch<double>(reinterpret_cast<const double *>(planes), I->d_cell_of, I->n_num, I->n, I->d_packed, I->stream);
+    else
+        pack_launch<float>(reinterpret_cast<const float *>(planes), I->d_cell_of, I->n_num, I->n, I->d_packed, I->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipMemcpyAsync(y, I->d_packed, sizeof(double) * 3 * I->n_num, hipMemcpyDeviceToHost, I->stream));
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return FKADJ_OK;
+}
+
+int fkadj_get_state(fkadj_instance *I, double *y)
+{
+    if (!I || !y) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    return pack_state(I, I->d_y0, y);
+}
+
+int fkadj_grad(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
+{
+    if (!I || !E) return FKADJ_ERR_ARG;
+    return eval_one(I, E, grad, Lc, L0, LE);
+}
+
+int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every, double *snaps, double *y_final,
+                  int advance)
+{
+    if (!I || !E || nsteps < 0 || nsteps > I->nt) return FKADJ_ERR_ARG;
+    if (snap_every > 0 && !snaps && nsteps / snap_every > 1) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    CHK(build_tables(I, E));
+    hipStream_t s = I->stream;
+    HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    const int nsnap = snap_every > 0 ? nsteps / snap_every - 1 : 0;   // FM:7-25: final state excluded
+    int taken = 0;
+    for (int i = 0; i < nsteps; i++) {
+        if (I->prec == FKADJ_F32) {
+            group_forward(I->group, MODE_FWD, i, 0, s);
+        } else {
+            auto pc = plain_ctx<double>(I);
+            plain_forward_step(pc, as<double>(fwd_loc(I, i)), as<double>(fwd_loc(I, i + 1)), i);
+        }
+        HIPCHK(hipGetLastError());
+        if (snap_every > 0 && (i + 1) % snap_every == 0 && taken < nsnap) {
+            CHK(pack_state(I, fwd_loc(I, i + 1), snaps + (size_t)taken * 3 * I->n_num));
+            taken++;
+        }
+    }
+    char *fin = fwd_loc(I, nsteps);
+    if (y_final) CHK(pack_state(I, fin, y_final));
+    if (advance) HIPCHK(hipMemcpyAsync(I->d_y0, fin, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return FKADJ_OK;
+}
+
+int fkadj_stimulate(fkadj_instance *I, const uint8_t *cell_mask, double value)
+{
+    if (!I || !cell_mask || !std::isfinite(value)) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    std::vector<uint8_t> m((size_t)I->n, 0);
+    for (int r = 0; r < I->n_num; r++) {
+        int k = I->cell_of[r];
+        int i = k / I->pitch, j = k % I->pitch;
+        m[k] = cell_mask[(size_t)i * I->cols + j] ? 1 : 0;
+    }
+    HIPCHK(hipMemcpyAsync(I->d_mask, m.data(), m.size(), hipMemcpyHostToDevice, I->stream));
+    if (I->prec == FKADJ_F64)
+        stimulate_launch<double>(as<double>(I->d_y0), I->d_mask, I->n, value, I->stream);
+    else
+        stimulate_launch<float>(as<float>(I->d_y0), I->d_mask, I->n, (float)value, I->stream);
+    HIPCHK(hipGetLastError());
+    HIPCHK(hipStreamSynchronize(I->stream));
+    return FKADJ_OK;
+}
+
+int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, const int32_t *active_index, int nrows,
+                   int ncols, int n_num, const double *gh_num, const double *gamma, const int64_t *S_colptr,
+                   const int64_t *S_rowind, const double *S_val, double alpha, double *L0, double *LE, double *Lcost,
+                   double *gradE, char *param_str, size_t param_str_cap)
+{
+    if (!Et || !y_ini || !active_index || !gh_num || !gamma || !S_colptr || !S_rowind || !S_val || !gradE)
+        return FKADJ_ERR_ARG;
+    fkadj_problem p{};
+    p.nrows = nrows;
+    p.ncols = ncols;
+    p.active_index = active_index;
+    p.gh_num = gh_num;
+    fkadj_model_reference(&p.model);
+    p.dt = dt;
+    p.nt = nt;
+    p.pulse_mode = FKADJ_PULSE_KNOTS;
+    p.alpha = alpha;
+    for (int v = 0; v < 3; v++) p.gamma[v] = gamma[v];
+    p.S_colptr = S_colptr;
+    p.S_rowind = S_rowind;
+    p.S_val = S_val;
+    p.precision = FKADJ_F64;
+    p.ckpt = 50;
+    fkadj_instance *I = nullptr;
+    int rc = fkadj_create(&p, &I);
+    if (rc) return rc;
+    if (I->n_num != n_num) { fkadj_destroy(I); return FKADJ_ERR_GEOMETRY; }   // G:1759-1762
+    rc = fkadj_set_state(I, y_ini);
+    if (!rc) rc = fkadj_grad(I, Et, gradE, Lcost, L0, LE);
+    fkadj_destroy(I);
+    if (!rc && param_str && param_str_cap) fkadj_param_string(&p.model, param_str, param_str_cap);
+    return rc;
+}
+
+int fkadj_set_profiling(int on)
+{
+    g_profiling.store(on ? 1 : 0);
+    return FKADJ_OK;
+}
+
+int fkadj_last_timing(fkadj_timing *t)
+{
+    if (!t) return FKADJ_ERR_ARG;
+    std::lock_guard<std::mutex> lk(g_timing_mu);
+    *t = g_timing;
+    return FKADJ_OK;
+}
+
+int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
+                     double *Lcost, double *L0, double *LE)
+{
+    if (count < 1 || !insts || !E) return FKADJ_ERR_ARG;
+    for (int k = 0; k < count; k++) {
+        fkadj_instance *I = insts[k];
+        if (!I || !E[k]) return FKADJ_ERR_ARG;
+        if (I->prec != FKADJ_F32 || I->nt != insts[0]->nt || I->K != insts[0]->K) return FKADJ_ERR_ARG;
+        for (int q = 0; q < k; q++)
+            if (insts[q] == I) return FKADJ_ERR_ARG;
+        if (!I->state_set) return FKADJ_ERR_STATE;
+        if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    }
+    hipStream_t s = insts[0]->stream;
+    for (int k = 0; k < count; k++) {
+        CHK(build_tables(insts[k], E[k]));
+        HIPCHK(hipStreamSynchronize(insts[k]->stream));
+    }
+    std::vector<fkadj_instance *> members(insts, insts + count);
+    FusedGroupDev *g = nullptr;
+    CHK(group_build(members, s, &g));
+    Sampler smp(g_profiling.load() != 0);
+    hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
+    int rc = hipEventRecord(e0, s) == hipSuccess ? run_fused(g, s, smp) : FKADJ_ERR_DEVICE;
+    if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = FKADJ_ERR_DEVICE;
+    int64_t cells = 0;
+    for (int k = 0; k < count && !rc; k++) {
+        rc = fetch_and_assemble(insts[k], s, E[k], grad ? grad[k] : nullptr, Lcost ? Lcost + k : nullptr,
+                                L0 ? L0 + k : nullptr, LE ? LE + k : nullptr);
+        cells += (int64_t)insts[k]->rows * insts[k]->pitch * insts[k]->nt;
+    }
+    if (!rc) record_timing(smp, e0, e1, cells);
+    (void)hipStreamSynchronize(s);
+    group_free(g);
+    return rc;
+}
+
+}  // extern "C"
+

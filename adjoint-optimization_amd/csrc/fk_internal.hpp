@@ -1,0 +1,101 @@
+// fk_internal.hpp -- shared declarations of the kernels and the host engine (not installed).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "fk_model.hpp"
+
+namespace fk {
+
+constexpr int PLAIN_BX = 64;   // one wavefront per 64 contiguous cells of a row
+constexpr int PLAIN_BY = 4;
+
+struct GridArgs {
+    int rows, pitch, n;        // n = rows * pitch; pitch is ncols rounded up to 64 (extra columns are holes)
+    const uint16_t *code;
+};
+
+// Deterministic FP64 sum over a 256-thread block (wavefront shuffles, then 4 partials in order).
+__device__ inline double block_sum_256(double v, int tid)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    __shared__ double part[4];
+    if ((tid & 63) == 0) part[tid >> 6] = v;
+    __syncthreads();
+    double s = 0.0;
+    if (tid == 0) s = ((part[0] + part[1]) + part[2]) + part[3];
+    return s;
+}
+
+// ---- plain path context
+template <typename T, typename M>
+struct PlainCtx {
+    GridArgs g;
+    M m;
+    double dt;
+    const double *Ef, *Ea;          // stage-time field tables, 3 per step
+    T *w0, *w1, *w2, *w3, *w4, *w5; // scratch planes (3n each)
+    hipStream_t stream;
+};
+
+template <typename T, typename M>
+void plain_forward_step(const PlainCtx<T, M> &c, const T *y_in, T *y_out, int step);
+template <typename T, typename M>
+void plain_adjoint_step(const PlainCtx<T, M> &c, const T *yA, const T *yB, const T *z_in, T *z_out, int row,
+                        double *partial_row);
+template <typename T, typename M>
+void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
+
+// ---- fused FP32 path: one launch per time step over a tile work-list of one or more instances
+constexpr int FUSED_TX = 64;    // tile width: one wavefront row
+constexpr int FUSED_NT = 256;   // threads per workgroup
+constexpr int FUSED_HALO = 4;   // one cell per RK stage
+
+struct FusedSlot {
+    int rows, pitch, n;
+    int nt, K, nck;             // steps, checkpoint interval, checkpoints
+    int ntiles;                 // tiles of this instance (zgh partial row length)
+    const uint16_t *code;
+    float *ck;                  // nck states: ck[c] = state at step min(c*K, nt)
+    float *seg;                 // K+1 states of the segment being recomputed (seg[0] unused)
+    float *ring;                // 2 states for the forward sweep between checkpoints
+    float *zb;                  // 2 adjoint states (ping-pong)
+    const double *Ef, *Ea;      // stage field tables, 3 per step
+    double *partial;            // (nt) rows x ntiles: zgh partials of adjoint steps
+    Model32 m;
+    float dt;
+};
+
+struct TileTask {
+    int slot, x0, y0, tile;     // instance, tile origin (cells), tile index within the instance
+};
+
+enum FusedMode { MODE_FWD = 0, MODE_REC = 1 };
+
+// Launchers (stream-ordered, no synchronisation).
+//   forward step `step` (0-based) of a sweep: MODE_FWD reads/writes ring/checkpoints,
+//   MODE_REC recomputes inside segment `seg_a` (first step of the segment) into seg[].
+void fused_forward_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          int mode, int step, int seg_a, hipStream_t stream);
+//   adjoint step over [t_{i-1}, t_i] for global i (seg_a = first step of the current segment).
+void fused_adjoint_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          int i, int seg_a, hipStream_t stream);
+
+// ---- common small kernels (fk_kernels_common.hip)
+// z (terminal adjoint, T planes) and L0 partials from y(T): CSR S over packed indices or s_diag*I.
+template <typename T>
+void objective_launch(int n_num, const int64_t *rowptr, const int32_t *colind, const double *val, double s_diag,
+                      const int32_t *cell_of, int n, const T *y, const double *gamma3, T *z, double *L0_part,
+                      hipStream_t stream);
+int objective_blocks(int n_num);
+// out[r] = sum_{b < nb} part[r*nb + b] in increasing b, r < nrows_out.
+void sum_rows_launch(const double *part, int nb, int nrows_out, double *out, hipStream_t stream);
+template <typename T>
+void pack_launch(const T *planes, const int32_t *cell_of, int n_num, int n, double *packed, hipStream_t stream);
+template <typename T>
+void unpack_launch(const double *packed, const int32_t *cell_of, int n_num, int n, T *planes, hipStream_t stream);
+template <typename T>
+void stimulate_launch(T *planes, const uint8_t *mask, int n, T value, hipStream_t stream);
+
+}  // namespace fk

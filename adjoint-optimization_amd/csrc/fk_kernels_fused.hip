@@ -1,0 +1,419 @@
+// fk_kernels_fused.hip -- FP32 fused time-step kernels over a tile work-list (one or many instances).
+//
+// One launch advances every listed tile by one full time step:
+//   forward  : the four RK4 stages of G:460-563 (G = src/cuda/fk_grad_Et_mono_PARAMETERS.cu)
+//   adjoint  : Hermite midpoint (G:1244-1301), the four adjoint RK4 stages (G:1349-1427) and the
+//              z.gh partial (G:1457-1483)
+// A 64 x TY output tile is staged in LDS with a 4-cell halo (one cell per RK stage); stage k is
+// evaluated on the tile grown by 4-k cells, so the tile needs no neighbour communication inside a
+// step and HBM sees one read of the inputs and one write of the outputs per step.
+// Work split: thread t owns the tile cells t + 256*s (one wavefront = one 64-cell row) and, in each
+// halo ring at distance d = 1..3 from the tile, the ring cell with perimeter index t.  The
+// ownership is the same in every stage, so per-cell values (state, RK increments, Jacobians) stay
+// in registers across the stages; LDS only holds the values neighbours read.
+#include "fk_internal.hpp"
+
+namespace fk {
+
+template <int TY>
+struct Tile {
+    static constexpr int TX = FUSED_TX, H = FUSED_HALO, NT = FUSED_NT;
+    static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+    static constexpr int TS = TX * TY / NT;          // tile cells per thread
+    static constexpr int NSLOT = TS + 3;              // + one cell in each of the 3 halo rings
+    static_assert(TX * TY % NT == 0, "tile must split evenly");
+    static_assert(2 * (TX + 6) + 2 * (TY + 4) <= NT, "ring 3 must fit one cell per thread");
+    __device__ static constexpr int ring_count(int d) { return 2 * (TX + 2 * d) + 2 * (TY + 2 * d - 2); }
+};
+
+// Cell of slot `sl` of thread `tid` (LDS coordinates); returns false if the thread owns no
+// cell in that ring.
+template <int TY>
+__device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
+{
+    using G = Tile<TY>;
+    if (sl < G::TS) {
+        int lin = tid + G::NT * sl;
+        ly = G::H + lin / G::TX;
+        lx = G::H + lin % G::TX;
+        return true;
+    }
+    const int d = sl - G::TS + 1;
+    const int W = G::TX + 2 * d, Hh = TY + 2 * d, base = G::H - d;
+    const int e = tid;
+    if (e >= G::ring_count(d)) return false;
+    if (e < W) { ly = base; lx = base + e; }
+    else if (e < 2 * W) { ly = G::H + TY + d - 1; lx = base + e - W; }
+    else {
+        int e2 = e - 2 * W;
+        if (e2 < Hh - 2) { ly = base + 1 + e2; lx = base; }
+        else { ly = base + 1 + e2 - (Hh - 2); lx = G::H + G::TX + d - 1; }
+    }
+    return true;
+}
+
+// Slot sl takes part in stage k (1..4): tile cells always, ring d while d <= 4-k.
+template <int TY>
+__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < Tile<TY>::TS || (sl - Tile<TY>::TS + 1) <= 4 - k; }
+
+template <int RY, int RX>
+__device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], uint32_t c, int ly, int lx, float E,
+                               float gh, float *f, float *yc)
+{
+    float l[3];
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        float ctr = s[v][ly][lx];
+        yc[v] = ctr;
+        l[v] = lap_val<float>(c, ctr, s[v][ly - 1][lx], s[v][ly + 1][lx], s[v][ly][lx - 1], s[v][ly][lx + 1], m.lc[v]);
+    }
+    Gates32 g = gates32(m, yc[0]);
+    float fu, fv, fw;
+    fion32(m, g, yc[0], yc[1], yc[2], fu, fv, fw);
+    f[0] = l[0] + fu + E * gh;
+    f[1] = l[1] + fv;
+    f[2] = l[2] + fw;
+}
+
+template <int RY, int RX>
+__device__ inline void lapT_lds(const Model32 &m, const float (*s)[RY][RX], uint32_t c, int ly, int lx, float *lt,
+                                float *zc)
+{
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        float ctr = s[v][ly][lx];
+        zc[v] = ctr;
+        lt[v] = lapT_val<float>(c, ctr, s[v][ly - 1][lx], s[v][ly + 1][lx], s[v][ly][lx - 1], s[v][ly][lx + 1], m.lc[v]);
+    }
+}
+
+__device__ inline void jtz(const Jac32 &J, const float *z, float *o)
+{
+    o[0] = J.a00 * z[0] + J.a01 * z[1] + J.a02 * z[2];
+    o[1] = J.a10 * z[0] + J.a11 * z[1];
+    o[2] = J.a20 * z[0] + J.a22 * z[2];
+}
+
+template <int TY, int RY, int RX>
+__device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
+                                   float (*dst)[RY][RX], uint16_t (*sc)[RX])
+{
+    const int gx0 = x0 - FUSED_HALO, gy0 = y0 - FUSED_HALO;
+    for (int idx = tid; idx < RX * RY; idx += FUSED_NT) {
+        int ly = idx / RX, lx = idx - ly * RX;
+        int gy = gy0 + ly, gx = gx0 + lx;
+        bool in = (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
+        int k = gy * S.pitch + gx;
+        if (sc) sc[ly][lx] = in ? S.code[k] : (uint16_t)0;
+#pragma unroll
+        for (int v = 0; v < 3; v++) dst[v][ly][lx] = in ? src[v * S.n + k] : 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------ forward
+template <int TY>
+__global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restrict__ slots,
+                                                        const TileTask *__restrict__ tasks, int mode, int step,
+                                                        int seg_a)
+{
+    using G = Tile<TY>;
+    constexpr int RX = G::RX, RY = G::RY, NS = G::NSLOT;
+    __shared__ float sY[3][RY][RX];
+    __shared__ uint16_t sC[RY][RX];
+
+    const TileTask t = tasks[blockIdx.x];
+    const FusedSlot &S = slots[t.slot];
+    const int tid = threadIdx.x;
+    const size_t st = (size_t)3 * S.n;
+    const float *yin;
+    float *yout;
+    if (mode == MODE_FWD) {
+        yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
+        int nx = step + 1;
+        yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
+                                             : S.ring + (size_t)(nx & 1) * st;
+    } else {
+        int j = step - seg_a;
+        yin = (j == 0) ? S.ck + (size_t)(seg_a / S.K) * st : S.seg + (size_t)j * st;
+        yout = S.seg + (size_t)(j + 1) * st;
+    }
+    const Model32 m = S.m;
+    const float dt = S.dt;
+    const float E1 = (float)S.Ef[3 * step], E23 = (float)S.Ef[3 * step + 1], E4 = (float)S.Ef[3 * step + 2];
+
+    load_region<TY, RY, RX>(S, yin, t.x0, t.y0, tid, sY, sC);
+    __syncthreads();
+
+    int ly[NS], lx[NS];
+    bool own[NS];
+    uint32_t cc[NS];
+    float y[NS][3], acc[G::TS][3], f[NS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        own[sl] = slot_cell<TY>(sl, tid, ly[sl], lx[sl]);
+        if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
+        cc[sl] = sC[ly[sl]][lx[sl]];
+#pragma unroll
+        for (int v = 0; v < 3; v++) y[sl][v] = sY[v][ly[sl]][lx[sl]];
+    }
+
+#pragma unroll
+    for (int k = 1; k <= 4; k++) {
+        const float E = (k == 1) ? E1 : (k == 4 ? E4 : E23);
+#pragma unroll
+        for (int sl = 0; sl < NS; sl++) {
+            if (!slot_in_stage<TY>(sl, k)) continue;
+            f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
+            if (own[sl] && (cc[sl] & C_ACTIVE)) {
+                float yc[3];
+                rhs_lds<RY, RX>(m, sY, cc[sl], ly[sl], lx[sl], E, m.ghc * (float)code_gh(cc[sl]), f[sl], yc);
+            }
+        }
+        if (k < 4) {
+            const float a = (k == 3) ? dt : 0.5f * dt;
+            const float b = (k == 1) ? dt * (1.0f / 6.0f) : dt * (1.0f / 3.0f);
+            __syncthreads();
+#pragma unroll
+            for (int sl = 0; sl < NS; sl++) {
+                if (!slot_in_stage<TY>(sl, k) || !own[sl]) continue;
+#pragma unroll
+                for (int v = 0; v < 3; v++) sY[v][ly[sl]][lx[sl]] = y[sl][v] + a * f[sl][v];
+                if (sl < G::TS) {
+#pragma unroll
+                    for (int v = 0; v < 3; v++) acc[sl][v] = (k == 1) ? b * f[sl][v] : acc[sl][v] + b * f[sl][v];
+                }
+            }
+            __syncthreads();
+        }
+    }
+    const float b4 = dt * (1.0f / 6.0f);
+#pragma unroll
+    for (int sl = 0; sl < G::TS; sl++) {
+        int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+        if (gy >= S.rows) continue;
+        int kk = gy * S.pitch + gx;
+#pragma unroll
+        for (int v = 0; v < 3; v++) yout[v * S.n + kk] = (cc[sl] & C_ACTIVE) ? y[sl][v] + (acc[sl][v] + b4 * f[sl][v]) : 0.0f;
+    }
+}
+
+// ------------------------------------------------------------------------------ adjoint
+template <typename T>
+__device__ inline T h3(T x0, T x1, T m0, T m1)
+{
+    constexpr double s = 1.0 / 3.0;
+    return T((s - 1.0) * (s - 1.0) * (2.0 * s + 1.0)) * x0 + T(s * (s - 1.0) * (s - 1.0)) * m0 +
+           T(s * s * (3.0 - 2.0 * s)) * x1 + T(s * s * (s - 1.0)) * m1;
+}
+template <typename T>
+__device__ inline T h4p(T x0, T x1, T m0, T m1)
+{
+    constexpr double s = 0.5;
+    return T((1.0 + 2.0 * s + 9.0 * s * s) * (s - 1.0) * (s - 1.0)) * x0 + T(s * (s - 1.0) * (s - 1.0)) * m0 +
+           T(-s * s * (6.0 - 16.0 * s + 9.0 * s * s)) * x1 + T((s - 1.0) * (9.0 * s - 5.0) * s * s / 4.0) * m1;
+}
+
+template <int TY>
+__global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restrict__ slots,
+                                                        const TileTask *__restrict__ tasks, int i, int seg_a)
+{
+    using G = Tile<TY>;
+    constexpr int RX = G::RX, RY = G::RY, NS = G::NSLOT, TS = G::TS;
+    __shared__ float sA[3][RY][RX];
+    __shared__ float sB[3][RY][RX];
+    __shared__ uint16_t sC[RY][RX];
+
+    const TileTask t = tasks[blockIdx.x];
+    const FusedSlot &S = slots[t.slot];
+    const int tid = threadIdx.x;
+    const size_t st = (size_t)3 * S.n;
+    const int segi = seg_a / S.K;
+    const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
+    const float *yA = (i - 1 == seg_a) ? S.ck + (size_t)segi * st : S.seg + (size_t)(i - 1 - seg_a) * st;
+    const float *yB = (i == b_end) ? S.ck + (size_t)(segi + 1) * st : S.seg + (size_t)(i - seg_a) * st;
+    const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
+    float *zout = S.zb + (size_t)((S.nt - i + 1) & 1) * st;
+    const Model32 m = S.m;
+    const float dt = S.dt;
+    const int row = i - 1;
+    const float EA = (float)S.Ea[3 * row], EB = (float)S.Ea[3 * row + 1], E3 = (float)S.Ea[3 * row + 2];
+
+    load_region<TY, RY, RX>(S, yA, t.x0, t.y0, tid, sA, sC);
+    load_region<TY, RY, RX>(S, yB, t.x0, t.y0, tid, sB, (uint16_t(*)[RX]) nullptr);
+    __syncthreads();
+
+    int ly[NS], lx[NS];
+    bool own[NS], act[NS];
+    uint32_t cc[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        own[sl] = slot_cell<TY>(sl, tid, ly[sl], lx[sl]);
+        if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
+        cc[sl] = sC[ly[sl]][lx[sl]];
+        act[sl] = own[sl] && (cc[sl] & C_ACTIVE);
+    }
+
+    // phase 1a: end slopes, y(t_{i-1}+dt/3) and the partial midpoint value on R1
+    float y3[NS][3], y4[NS][3], yb[NS][3], ya[TS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+#pragma unroll
+        for (int v = 0; v < 3; v++) { y3[sl][v] = 0.0f; y4[sl][v] = 0.0f; yb[sl][v] = 0.0f; }
+        if (sl < TS) { ya[sl][0] = ya[sl][1] = ya[sl][2] = 0.0f; }
+        if (act[sl]) {
+            const float gh = m.ghc * (float)code_gh(cc[sl]);
+            float fa[3], fb[3], a[3];
+            rhs_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], EA, gh, fa, a);
+            rhs_lds<RY, RX>(m, sB, cc[sl], ly[sl], lx[sl], EB, gh, fb, yb[sl]);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float m0 = fa[v] * dt, m1 = fb[v] * dt;
+                y3[sl][v] = h3<float>(a[v], yb[sl][v], m0, m1);
+                y4[sl][v] = h4p<float>(a[v], yb[sl][v], m0, m1);
+                if (sl < TS) ya[sl][v] = a[v];
+            }
+        }
+    }
+    __syncthreads();
+    // y(1/3) -> sB (R1), z -> sA (whole region)
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++)
+        if (own[sl])
+#pragma unroll
+            for (int v = 0; v < 3; v++) sB[v][ly[sl]][lx[sl]] = y3[sl][v];
+    load_region<TY, RY, RX>(S, zin, t.x0, t.y0, tid, sA, (uint16_t(*)[RX]) nullptr);
+    __syncthreads();
+
+    // phase 1b: adjoint stage 1 on R1 with J(y_i)
+    float z[NS][3], Zk[NS][3], dz[TS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        float lt[3], o[3];
+        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
+        z[sl][0] = z[sl][1] = z[sl][2] = 0.0f;
+        if (sl < TS) dz[sl][0] = dz[sl][1] = dz[sl][2] = 0.0f;
+        if (!own[sl]) continue;
+        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, z[sl]);
+        if (act[sl]) {
+            Gates32 g = gates32(m, yb[sl][0]);
+            Jac32 J = jac32(m, g, yb[sl][0], yb[sl][1], yb[sl][2]);
+            jtz(J, z[sl], o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[sl][v] = z[sl][v] + 0.5f * dt * fz;
+                if (sl < TS) dz[sl][v] = (1.0f / 6.0f) * dt * fz;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++)
+        if (own[sl])
+#pragma unroll
+            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
+    __syncthreads();
+
+    // phase 2: midpoint state yh, J(yh), adjoint stage 2 on R2
+    Jac32 Jh[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<TY>(sl, 2)) continue;
+        float lt[3], zc[3], o[3];
+        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
+        if (!own[sl]) continue;
+        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            float f3[3], yc[3], yh[3];
+            rhs_lds<RY, RX>(m, sB, cc[sl], ly[sl], lx[sl], E3, m.ghc * (float)code_gh(cc[sl]), f3, yc);
+#pragma unroll
+            for (int v = 0; v < 3; v++) yh[v] = y4[sl][v] + (float)(27.0 / 64.0) * (f3[v] * dt);
+            Gates32 g = gates32(m, yh[0]);
+            Jh[sl] = jac32(m, g, yh[0], yh[1], yh[2]);
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[sl][v] = z[sl][v] + 0.5f * dt * fz;
+                if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++)
+        if (slot_in_stage<TY>(sl, 2) && own[sl])
+#pragma unroll
+            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
+    __syncthreads();
+
+    // phase 3: adjoint stage 3 on R3 with J(yh)
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<TY>(sl, 3)) continue;
+        float lt[3], zc[3], o[3];
+        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
+        if (!own[sl]) continue;
+        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[sl][v] = z[sl][v] + dt * fz;
+                if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++)
+        if (slot_in_stage<TY>(sl, 3) && own[sl])
+#pragma unroll
+            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
+    __syncthreads();
+
+    // phase 4: adjoint stage 4 on the tile with J(y_{i-1}); new z and its z.gh partial
+    double part = 0.0;
+#pragma unroll
+    for (int sl = 0; sl < TS; sl++) {
+        float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            Gates32 g = gates32(m, ya[sl][0]);
+            Jac32 J = jac32(m, g, ya[sl][0], ya[sl][1], ya[sl][2]);
+            jtz(J, zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) zn[v] = z[sl][v] + (dz[sl][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
+        }
+        int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+        if (gy < S.rows) {
+            int kk = gy * S.pitch + gx;
+#pragma unroll
+            for (int v = 0; v < 3; v++) zout[v * S.n + kk] = zn[v];
+            int gh = code_gh(cc[sl]);
+            if (act[sl] && gh != 0) part += (double)zn[0] * ((double)m.ghc * (double)gh);
+        }
+    }
+    double s = block_sum_256(part, tid);
+    if (tid == 0) S.partial[(size_t)row * S.ntiles + t.tile] = s;
+}
+
+// ------------------------------------------------------------------------------ launchers
+void fused_forward_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          int mode, int step, int seg_a, hipStream_t stream)
+{
+    if (tile_rows == 32)
+        hipLaunchKernelGGL(k_fused_fwd<32>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, mode, step, seg_a);
+    else
+        hipLaunchKernelGGL(k_fused_fwd<16>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, mode, step, seg_a);
+}
+
+void fused_adjoint_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
+                          int seg_a, hipStream_t stream)
+{
+    (void)tile_rows;  // the adjoint keeps 16-row tiles: its register footprint doubles with 32
+    hipLaunchKernelGGL(k_fused_adj<16>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, i, seg_a);
+}
+
+}  // namespace fk

@@ -1,0 +1,160 @@
+/*
+ * fkadj.h -- C ABI of the MI355X Fenton-Karma adjoint-gradient library (libfkadj.so).
+ *
+ * Plain C: pointers, sizes and error codes; no C++ or PyTorch types cross this boundary.
+ * All host arrays are caller-owned; the library copies what it needs.  Every entry point
+ * returns FKADJ_OK (0) or a negative error code and never exits the process (the reference
+ * aborts the MATLAB process on any error, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1488-1512).
+ *
+ * Reference interfaces replaced (paths relative to the reference checkout):
+ *   fk_grad_Et_comp          src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1572  -> fkadj_grad_ref
+ *   mexFunction (gradient)   src/cuda/fk_grad_Et_param_mex.cpp:11-92           -> INTEGRATION.md stub
+ *   fk_forward_Et_comp       src/cuda/fk_forward_Et_mono_PARAMETERS.cu:646     -> fkadj_forward
+ *   compile-time constants   src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24 -> fkadj_model
+ *   parameters_string        src/cuda/fk_parameters_string_00_05_alpha_0p8_dx_0p035.cpp:1-32
+ *                                                                              -> fkadj_param_string
+ * Grid convention (src/cuda/fk_grad_Et_mono_PARAMETERS.cu:25-28): cell k = i*ncols + j, j contiguous.
+ * active_index[k] is the 1-based packed index of cell k, 0 for a hole; packed state vectors are
+ * [u; v; w], each n_num long, in increasing k (src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1533-1551).
+ */
+#ifndef FKADJ_H
+#define FKADJ_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- error codes */
+#define FKADJ_OK             0
+#define FKADJ_ERR_ARG       -1  /* bad argument (null pointer, size, mode) */
+#define FKADJ_ERR_GEOMETRY  -2  /* grid/mask invalid: a cell with both axial neighbours missing,
+                                   S size differs from the active-cell count (reference: exit) */
+#define FKADJ_ERR_NOMEM     -3  /* device allocation failed or admission over the memory budget */
+#define FKADJ_ERR_DEVICE    -4  /* HIP runtime error (no GPU, launch failure, ...) */
+#define FKADJ_ERR_NONFINITE -5  /* NaN/Inf in an input or a result */
+#define FKADJ_ERR_STATE     -6  /* call out of order (e.g. gradient before a state was set) */
+#define FKADJ_ERR_CANCELLED -7  /* request cancelled before completion */
+#define FKADJ_ERR_BUSY      -8  /* no result ready yet (serving poll) */
+#define FKADJ_ERR_NOTFOUND  -9  /* unknown request id */
+
+const char *fkadj_strerror(int code);
+
+/* ---------------------------------------------------------------- model */
+/* FK constants, diffusion coefficients (cm^2/ms), smoothing ksm and mesh width h (cm).
+ * Field order and values of fkadj_model_reference() are those of
+ * src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24 (h = 0.035 there, compile-time). */
+typedef struct fkadj_model {
+    double u_c, u_v, tau_d, tau_v1_minus, tau_v2_minus, tau_v_plus, tau_0, tau_r, tau_si,
+        u_csi, k_fk, tau_w_minus, tau_w_plus, Du, Dv, Dw, ksm, h;
+} fkadj_model;
+
+void fkadj_model_reference(fkadj_model *m);
+
+/* Provenance text: the model constants in effect (replaces the compiled-in parameters_string,
+ * returned by the reference as a callee-allocated, unterminated buffer, G:1932-1935).
+ * Writes at most cap bytes including the terminating NUL; returns the full length. */
+int fkadj_param_string(const fkadj_model *m, char *buf, size_t cap);
+
+/* ---------------------------------------------------------------- reference-shaped entry */
+/* Drop-in for fk_grad_Et_comp (src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938):
+ *   same inputs and outputs; the sparse S is plain CSC over packed indices (n_num x n_num,
+ *   int64 column pointers and row indices) instead of a MATLAB mxArray; gh_num is the per-active
+ *   cell ghost_lr (double, as MATLAB passes it); the provenance string goes into a caller buffer.
+ *   E(t) is given at nt+1 knots t_i = i*dt and read at the stage times through a not-a-knot cubic
+ *   spline, as in the reference.  Runs the FP64 GPU path with the forward trajectory kept in HBM
+ *   as checkpoints every 50 steps (recomputed per segment in the backward sweep).
+ * Outputs: gradE[nt+1] = alpha*E_i - z_u(t_i).gh, L0 = sum_v gamma_v y_v^T S y_v at T,
+ *   LE = dt*sum E_i^2, Lcost = L0/2 + alpha*LE/2. */
+int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, const int32_t *active_index,
+                   int nrows, int ncols, int n_num, const double *gh_num, const double *gamma,
+                   const int64_t *S_colptr, const int64_t *S_rowind, const double *S_val, double alpha,
+                   double *L0, double *LE, double *Lcost, double *gradE, char *param_str, size_t param_str_cap);
+
+/* ---------------------------------------------------------------- problems and instances */
+#define FKADJ_F64 0            /* FP64 state, one kernel per RK stage ("plain" path) */
+#define FKADJ_F32 1            /* FP32 state, fused step kernels, FP64 reductions */
+
+#define FKADJ_PULSE_KNOTS     0  /* E given at nt+1 knots, spline at sub-step times (reference) */
+#define FKADJ_PULSE_PIECEWISE 1  /* n_amp piecewise-constant amplitudes over equal pieces of T;
+                                    E is constant over each time step; the gradient returned is
+                                    dJ/dA_k (trapezoid quadrature of alpha*E - z.gh per step) */
+
+typedef struct fkadj_problem {
+    int nrows, ncols;                 /* grid; any size >= 4 in each direction */
+    const int32_t *active_index;      /* nrows*ncols */
+    const double *gh_num;             /* n_num values of ghost_lr (-1, 0, 1); NULL: derived from the mask */
+    fkadj_model model;
+    double dt;                        /* ms */
+    int nt;                           /* time steps; T = nt*dt */
+    int pulse_mode;                   /* FKADJ_PULSE_* */
+    int n_amp;                        /* PIECEWISE: number of amplitudes, nt % n_amp == 0 */
+    double alpha;                     /* energy weight: Lcost = L0/2 + alpha*LE/2 */
+    double gamma[3];                  /* objective weights of u, v, w */
+    /* objective matrix over packed indices: CSC (n_num x n_num), or S = s_diag*I when S_colptr==NULL */
+    const int64_t *S_colptr;
+    const int64_t *S_rowind;
+    const double *S_val;
+    double s_diag;
+    int precision;                    /* FKADJ_F64 / FKADJ_F32 */
+    int ckpt;                         /* forward checkpoint interval K (steps); 0 -> 50 */
+    int tile_rows;                    /* FP32 fused tile height (16 or 32); 0 -> autotuned table */
+} fkadj_problem;
+
+typedef struct fkadj_instance fkadj_instance;
+
+/* Uploads geometry, codes and objective matrix; allocates the device state, checkpoint and
+ * segment buffers.  The initial state must be set before the first gradient. */
+int fkadj_create(const fkadj_problem *p, fkadj_instance **out);
+int fkadj_destroy(fkadj_instance *inst);
+
+/* Device bytes held by an instance with this problem (for admission by memory footprint). */
+int64_t fkadj_footprint_bytes(const fkadj_problem *p);
+
+/* y_packed: 3*n_num values [u; v; w].  Stays resident until replaced. */
+int fkadj_set_state(fkadj_instance *inst, const double *y_packed);
+int fkadj_get_state(fkadj_instance *inst, double *y_packed);
+int fkadj_n_active(const fkadj_instance *inst);
+int fkadj_n_controls(const fkadj_instance *inst); /* nt+1 (KNOTS) or n_amp (PIECEWISE) */
+
+/* One gradient evaluation from the resident initial state.
+ * E: fkadj_n_controls() values; grad: same length.  Any of the scalar outputs may be NULL. */
+int fkadj_grad(fkadj_instance *inst, const double *E, double *grad, double *Lcost, double *L0, double *LE);
+
+/* Forward solve from the resident state over nsteps (<= nt) with field E (same layout as for
+ * fkadj_grad).  With advance != 0 the final state replaces the resident state.
+ * snap_every > 0: snapshots of the packed state every snap_every steps (excluding step 0 and,
+ * as in src/cuda/fk_forward_Et_param_mex.cpp:7-25, excluding the final state) go to snaps,
+ * (nsteps/snap_every - 1) * 3*n_num values.  y_final (may be NULL): packed final state. */
+int fkadj_forward(fkadj_instance *inst, const double *E, int nsteps, int snap_every, double *snaps,
+                  double *y_final, int advance);
+
+/* Stimulus helper for building initial states: u = value on the cells where mask[k] != 0. */
+int fkadj_stimulate(fkadj_instance *inst, const uint8_t *cell_mask, double value);
+
+/* ---------------------------------------------------------------- batched evaluation */
+/* Gradients of `count` FP32 instances (any grid sizes, equal nt and ckpt) computed together:
+ * every forward, recompute and adjoint step of all instances is one kernel launch over a
+ * tile work-list of all instances.  Results equal the instances' own fkadj_grad bit for bit. */
+int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
+                     double *Lcost, double *L0, double *LE);
+
+/* ---------------------------------------------------------------- timing */
+typedef struct fkadj_timing {
+    double eval_ms;          /* last gradient evaluation, device time between HIP events */
+    double fwd_kernel_ms;    /* mean duration of one forward-step launch (all tiles of the call) */
+    double adj_kernel_ms;    /* mean duration of one adjoint-step launch */
+    int64_t fwd_launches, adj_launches;
+    int64_t cell_steps;      /* active+padding cells x steps processed per sweep */
+} fkadj_timing;
+
+/* Enables per-launch HIP events on the next calls (adds small overhead); 0 disables. */
+int fkadj_set_profiling(int on);
+int fkadj_last_timing(fkadj_timing *t);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* FKADJ_H */

@@ -1,0 +1,100 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared entry point, and
+argument / geometry validation returns error codes before any device work."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import fkadj
+from fkadj import _lib as L
+from fkadj import geometry as G
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h"):
+            src = open(os.path.join(ROOT, "include", fn)).read()
+            src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+            names |= set(re.findall(r"\b(fkadj_\w+)\s*\(", src))
+    return sorted(names)
+
+
+def test_library_exports_every_declared_symbol():
+    lib = fkadj.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in names if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reference_model_and_param_string():
+    import oracle as O
+    m = fkadj.reference_model()
+    om = O.reference_model()
+    for f in L.MODEL_FIELDS:
+        assert getattr(m, f) == getattr(om, f), f
+    s = fkadj.param_string(m)
+    for f in L.MODEL_FIELDS:
+        assert f"\n{f} " in s
+    assert s.endswith("\n") and "\0" not in s
+
+
+def test_strerror_codes():
+    for c in range(0, -10, -1):
+        assert fkadj.strerror(c)
+    assert "geometry" in fkadj.strerror(L.ERR_GEOMETRY)
+
+
+def _expect(code, **kw):
+    base = dict(dt=0.1, nt=10)
+    base.update(kw)
+    ai = base.pop("ai")
+    shape = base.pop("shape")
+    with pytest.raises(fkadj.FkadjError) as e:
+        fkadj.Instance(ai, *shape, **base)
+    assert e.value.code == code, (e.value.code, code)
+
+
+def test_validation_errors_before_device_work():
+    mask = np.ones((16, 16), bool)
+    ai = G.active_index_from_mask(mask)
+    # a cell with both axial neighbours missing (the reference's mirror rule gives NaN there)
+    bad = mask.copy()
+    bad[5, 4] = bad[5, 6] = False
+    _expect(L.ERR_GEOMETRY, ai=G.active_index_from_mask(bad), shape=(16, 16))
+    dup = ai.copy(); dup[3] = dup[4]
+    _expect(L.ERR_GEOMETRY, ai=dup, shape=(16, 16))
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), pulse_mode=L.PULSE_PIECEWISE, n_amp=3)      # 10 % 3 != 0
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), precision=7)
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), nt=2)                                       # spline needs 4 knots
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), ghost_lr=np.full(256, 0.5))
+    _expect(L.ERR_ARG, ai=ai, shape=(3, 16), dt=0.1)
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), dt=-1.0)
+
+
+def test_footprint_grows_with_size_and_precision():
+    p = L.Problem()
+    p.nrows = p.ncols = 256
+    p.nt, p.ckpt, p.precision = 1000, 50, L.F32
+    a = fkadj.footprint_bytes(p)
+    p.nrows = p.ncols = 512
+    b = fkadj.footprint_bytes(p)
+    p.precision = L.F64
+    c = fkadj.footprint_bytes(p)
+    assert 0 < a < b < c
+    # 512^2 FP32, 10^4 steps, K=50: about 201 checkpoints + 51 segment states of 3 MiB
+    p.precision, p.nt = L.F32, 10000
+    assert 0.7e9 < fkadj.footprint_bytes(p) < 1.2e9
+
+
+def test_disc_holes_have_no_isthmus_and_closed_form_ghost():
+    m = G.disc_holes(96, 128, 25, 2, 6, seed=7)
+    assert not G.isthmus_cells(m).any()
+    gh = G.ghost_lr_from_mask(m)
+    assert set(np.unique(gh)) <= {-1.0, 0.0, 1.0}
+    L_ = G.lap_matrix(m)
+    assert np.allclose(L_.sum(axis=1), 0.0)   # mirror Laplacian annihilates constants

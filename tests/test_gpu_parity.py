@@ -1,0 +1,222 @@
+"""GPU parity of libfkadj against the FP64 oracle and the reference's golden data (through the C ABI).
+
+Tolerances:
+  * FP64 GPU path vs oracle: relative 1e-9 (different tanh/FMA rounding, amplified by the dynamics);
+    vs the reference's golden gradE at 256^2 / 3000 steps: relative 1e-9.
+  * FP32 path vs FP64 oracle: relative 1e-4 on J and on the gradient (the target of the design).
+  * batched vs alone, checkpoint interval, tile height: bit-identical (same kernels, fixed order).
+"""
+import numpy as np
+import pytest
+
+import fkadj
+from fkadj import geometry as G
+
+from cases import crop_case, disc_case, golden, oracle_grad_knots, oracle_grad_piecewise, relerr
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+@pytest.fixture(scope="module")
+def crop():
+    return crop_case(64, 96, 32, 64)
+
+
+def _Et(nt, seed=0):
+    rng = np.random.default_rng(seed)
+    t = np.arange(nt + 1)
+    return 1.5 + np.sin(t * 0.15 + rng.uniform(0, 3)) + 0.3 * rng.standard_normal(nt + 1) * 0.1
+
+
+def test_fp64_ref_entry_matches_oracle(crop):
+    _print_case("fp64_ref_crop", shape=crop.mask.shape, nt=60)
+    d, meta = golden()
+    nt, dt = 60, 0.1
+    Et = _Et(nt)
+    S = G.S_matrix(crop.mask)
+    g_ref, Lc_ref, L0_ref, LE_ref = oracle_grad_knots(crop, dt, Et, meta["gamma"], meta["alpha"], S)
+    g, Lc, L0, LE, pstr = fkadj.fk_grad_Et_mex(Et, dt, nt, crop.y0, crop.ai, crop.nrows, crop.ncols, crop.gh,
+                                               meta["gamma"], S, meta["alpha"])
+    assert relerr(g, g_ref) < 1e-9
+    assert abs(Lc - Lc_ref) < 1e-9 * abs(Lc_ref)
+    assert abs(L0 - L0_ref) < 1e-9 * abs(L0_ref)
+    assert LE == pytest.approx(LE_ref, rel=1e-14)
+    assert "ksm" in pstr
+
+
+def test_fp32_matches_oracle_knots(crop):
+    _print_case("fp32_knots_crop", shape=crop.mask.shape, nt=60)
+    _, meta = golden()
+    nt, dt = 60, 0.1
+    Et = _Et(nt, 1)
+    S = G.S_matrix(crop.mask)
+    g_ref, Lc_ref, _, _ = oracle_grad_knots(crop, dt, Et, meta["gamma"], meta["alpha"], S)
+    with fkadj.Instance(crop.ai, crop.nrows, crop.ncols, dt=dt, nt=nt, ghost_lr=crop.gh, alpha=meta["alpha"],
+                        gamma=meta["gamma"], S=S, precision=fkadj.F32, ckpt=16) as inst:
+        inst.set_state(crop.y0)
+        g, Lc, _, _ = inst.grad(Et)
+    assert relerr(g, g_ref) < 1e-4
+    assert abs(Lc - Lc_ref) < 1e-4 * abs(Lc_ref)
+
+
+@pytest.mark.parametrize("shape,holes,seed", [((48, 128), 10, 3), ((70, 100), 14, 5)])
+def test_fp32_and_fp64_piecewise_disc_holes(shape, holes, seed):
+    """Config-style problem: h=0.025, dt=0.02, mean(u^2) + lambda*energy, piecewise amplitudes,
+    ragged grid (ncols not a multiple of 64, nrows not a multiple of the tile height)."""
+    _print_case("piecewise_disc", shape=shape, holes=holes, seed=seed)
+    case = disc_case(*shape, holes, seed)
+    m = fkadj.reference_model(h=0.025)
+    import oracle as O
+    om = O.reference_model(h=0.025)
+    nt, dt, K = 120, 0.02, 6
+    A = 10.0 * np.random.default_rng(seed).uniform(0, 1, K)
+    lam = 1e-3
+    S = G.mean_u2_S(case.n_act)
+    g_ref, Lc_ref, L0_ref, _ = oracle_grad_piecewise(case, dt, nt, A, (1, 0, 0), 2 * lam, S, om)
+    for prec, tol in ((fkadj.F64, 1e-9), (fkadj.F32, 1e-4)):
+        with fkadj.Instance(case.ai, case.nrows, case.ncols, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE,
+                            n_amp=K, alpha=2 * lam, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=prec,
+                            ckpt=25) as inst:
+            inst.set_state(case.y0)
+            g, Lc, L0, _ = inst.grad(A)
+        assert relerr(g, g_ref) < tol, (prec, relerr(g, g_ref))
+        assert abs(Lc - Lc_ref) < tol * abs(Lc_ref), (prec, Lc, Lc_ref)
+
+
+def test_fp32_checkpoint_and_tile_invariance(crop):
+    """The recomputation replays the forward kernels bit for bit, so K does not change the result;
+    the forward tile height changes only the halo bookkeeping, not the arithmetic."""
+    _print_case("fp32_invariance", shape=crop.mask.shape)
+    _, meta = golden()
+    nt, dt = 40, 0.1
+    Et = _Et(nt, 2)
+    S = G.S_matrix(crop.mask)
+    out = []
+    for ck, ty in ((1, 16), (7, 16), (40, 16), (13, 32)):
+        with fkadj.Instance(crop.ai, crop.nrows, crop.ncols, dt=dt, nt=nt, ghost_lr=crop.gh, alpha=meta["alpha"],
+                            gamma=meta["gamma"], S=S, precision=fkadj.F32, ckpt=ck, tile_rows=ty) as inst:
+            inst.set_state(crop.y0)
+            out.append(inst.grad(Et))
+    for o in out[1:]:
+        assert np.array_equal(o[0], out[0][0]) and o[1] == out[0][1]
+
+
+def test_batch_equals_alone():
+    """Uneven grids evaluated in shared launches give exactly the results of separate calls."""
+    _print_case("batch_vs_alone", sizes="(64,64),(96,128),(40,192)")
+    insts, Es, alone = [], [], []
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K = 60, 0.02, 6
+    for k, (H, W) in enumerate([(64, 64), (96, 128), (40, 192)]):
+        case = disc_case(H, W, 6 + 2 * k, 10 + k)
+        inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                              alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=20)
+        inst.set_state(case.y0)
+        A = np.random.default_rng(k).uniform(0, 10, K)
+        insts.append(inst)
+        Es.append(A)
+        alone.append(inst.grad(A))
+    mixed = fkadj.batch_grad(insts, Es)
+    for a, b in zip(alone, mixed):
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+    for inst in insts:
+        inst.close()
+
+
+def test_forward_snapshots_match_oracle(crop):
+    _print_case("forward_snapshots", shape=crop.mask.shape)
+    import oracle as O
+    nt, dt = 30, 0.1
+    Et = _Et(nt, 3)
+    Ef, _ = O.knot_tables(dt, Et)
+    om = O.reference_model()
+    with fkadj.Instance(crop.ai, crop.nrows, crop.ncols, dt=dt, nt=nt, ghost_lr=crop.gh, precision=fkadj.F64,
+                        S=G.S_matrix(crop.mask)) as inst:
+        inst.set_state(crop.y0)
+        snaps, yf = inst.forward(Et, snap_every=10)
+    assert snaps.shape[0] == 2   # steps 10 and 20; the final state is returned separately
+    y = crop.planes()
+    gh = crop.gh_plane(om)
+    for k, s in enumerate((10, 20, 30)):
+        yk = O.forward(om, y, gh, dt, Ef[:3 * s])
+        packed = np.concatenate([yk[v][crop.mask] for v in range(3)])
+        got = snaps[k] if k < 2 else yf
+        assert relerr(got, packed) < 1e-12
+
+
+def test_errors_fail_only_the_call(crop):
+    """Bad inputs come back as error codes; the instance stays usable."""
+    _print_case("errors")
+    nt, dt = 20, 0.1
+    S = G.S_matrix(crop.mask)
+    with fkadj.Instance(crop.ai, crop.nrows, crop.ncols, dt=dt, nt=nt, ghost_lr=crop.gh, S=S,
+                        precision=fkadj.F32) as inst:
+        with pytest.raises(fkadj.FkadjError) as e:
+            inst.grad(np.ones(nt + 1))
+        assert e.value.code == -6                    # no state yet
+        bad = crop.y0.copy(); bad[3] = np.nan
+        with pytest.raises(fkadj.FkadjError) as e:
+            inst.set_state(bad)
+        assert e.value.code == -5
+        inst.set_state(crop.y0)
+        E = np.ones(nt + 1); E[4] = np.inf
+        with pytest.raises(fkadj.FkadjError) as e:
+            inst.grad(E)
+        assert e.value.code == -5
+        g1 = inst.grad(np.ones(nt + 1))[0]
+        g2 = inst.grad(np.ones(nt + 1))[0]
+        assert np.array_equal(g1, g2)
+
+
+@pytest.mark.slow
+def test_golden_256_iteration50_fp64():
+    """The reference's own case: 256^2, 3000 steps, spiral state, iterate 50 -> stored gradE/Lcost."""
+    _print_case("golden256_it50_fp64")
+    d, meta = golden()
+    mask = G.mask_from_active_index(d["active_index"], 256, 256)
+    S = G.S_matrix(mask)
+    g, Lc, L0, LE, _ = fkadj.fk_grad_Et_mex(d["Et_50"], meta["dt"], 3000, d["y0"], d["active_index"], 256, 256,
+                                            d["ghost_lr"].astype(float), meta["gamma"], S, meta["alpha"])
+    assert relerr(g, d["gradE_50"]) < 1e-9
+    assert abs(Lc - meta["iterations"]["50"]["Lcost"]) < 1e-10 * meta["iterations"]["50"]["Lcost"]
+
+
+@pytest.mark.slow
+def test_golden_256_gradient_descent_iterations_1_to_3():
+    """Three steps of the reference's driver (E <- E - ds*gradE, src/cuda/simple_grad_desc.m:103)
+    reproduce the stored Lcost/L0/LE histories of iterations 1-3."""
+    _print_case("golden256_gd3_fp64")
+    d, meta = golden()
+    mask = G.mask_from_active_index(d["active_index"], 256, 256)
+    S = G.S_matrix(mask)
+    with fkadj.Instance(d["active_index"], 256, 256, dt=meta["dt"], nt=3000, ghost_lr=d["ghost_lr"].astype(float),
+                        alpha=meta["alpha"], gamma=meta["gamma"], S=S, precision=fkadj.F64) as inst:
+        inst.set_state(d["y0"])
+        E = d["Et_seed"].copy()
+        for it in range(3):
+            g, Lc, L0, LE = inst.grad(E)
+            assert abs(Lc - d["Lcost_vec_50"][it]) < 1e-10 * d["Lcost_vec_50"][it]
+            assert abs(L0 - d["L0_vec_50"][it]) < 1e-10 * d["L0_vec_50"][it]
+            assert abs(LE - d["LE_vec_50"][it]) < 1e-12 * d["LE_vec_50"][it]
+            E = E - meta["ds"] * g
+
+
+@pytest.mark.slow
+def test_golden_256_fp32_error_vs_fp64():
+    """FP32 fused path on the reference's case: J and gradE within 1e-4 of the FP64 golden values."""
+    _print_case("golden256_it50_fp32")
+    d, meta = golden()
+    mask = G.mask_from_active_index(d["active_index"], 256, 256)
+    S = G.S_matrix(mask)
+    with fkadj.Instance(d["active_index"], 256, 256, dt=meta["dt"], nt=3000, ghost_lr=d["ghost_lr"].astype(float),
+                        alpha=meta["alpha"], gamma=meta["gamma"], S=S, precision=fkadj.F32) as inst:
+        inst.set_state(d["y0"])
+        g, Lc, _, _ = inst.grad(d["Et_50"])
+    e = relerr(g, d["gradE_50"])
+    print(f"fp32 golden gradE rel err {e:.3e}, Lcost rel err {abs(Lc / meta['iterations']['50']['Lcost'] - 1):.3e}")
+    assert e < 1e-4
+    assert abs(Lc - meta["iterations"]["50"]["Lcost"]) < 1e-4 * meta["iterations"]["50"]["Lcost"]

@@ -170,7 +170,7 @@ void fill_models(fkadj_instance *I)
     const double L2E = 1.4426950408889634;
     b.kq = (float)(-2.0 * md.ksm * L2E);
     b.bq = (float)(2.0 * md.ksm * md.u_v * L2E);
-    b.cpq = (float)std::exp(-2.0 * md.ksm * (md.u_c - md.u_v));
+    b.cpq = (float)std::exp(2.0 * md.ksm * (md.u_c - md.u_v));
     b.ks = (float)(-2.0 * md.k_fk * L2E);
     b.bs = (float)(2.0 * md.k_fk * md.u_csi * L2E);
     b.two_ksm = (float)(2.0 * md.ksm);

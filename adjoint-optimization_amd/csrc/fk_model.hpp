@@ -38,7 +38,7 @@ struct Model64 {
 
 struct Model32 {
     float kq, bq;        // exp2(kq*u + bq) = exp(-2 ksm (u - u_v))
-    float cpq;           // exp(-2 ksm (u_c - u_v)):  e_p = e_q * cpq
+    float cpq;           // exp(2 ksm (u_c - u_v)):  e_p = exp(-2 ksm (u - u_c)) = e_q * cpq
     float ks, bs;        // exp2(ks*u + bs) = exp(-2 k (u - u_csi))
     float two_ksm, two_k;
     float u_c, one_plus_uc;
@@ -103,12 +103,14 @@ struct Gates32 {
 __device__ inline float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
+// The exponents are capped at 2^64 so that e_p = e_q * cpq stays finite (then p ~ 1e-27 and
+// 1-p = e_p p = 1 to FP32 precision, instead of inf * 0).
 __device__ inline Gates32 gates32(const Model32 &m, float u)
 {
     Gates32 g;
-    float eq = fast_exp2(fmaf(m.kq, u, m.bq));
+    float eq = fast_exp2(fminf(fmaf(m.kq, u, m.bq), 64.0f));
     float ep = eq * m.cpq;
-    float es = fast_exp2(fmaf(m.ks, u, m.bs));
+    float es = fast_exp2(fminf(fmaf(m.ks, u, m.bs), 64.0f));
     g.q = fast_rcp(1.0f + eq);
     g.omq = eq * g.q;
     g.p = fast_rcp(1.0f + ep);

@@ -106,7 +106,8 @@ def grad_matrices(mask: np.ndarray):
     """Per-axis second-order finite differences D_i, D_j over active cells (grid units).
 
     Central ``(y+ - y-)/2`` when both neighbours exist, otherwise the one-sided
-    ``(-3 y0 + 4 y1 - y2)/2`` pointing into the tissue (sign mirrored for the - side).
+    ``(-3 y0 + 4 y1 - y2)/2`` pointing into the tissue (sign mirrored for the - side), or
+    ``y1 - y0`` when the open side has a single cell.
     """
     m = np.asarray(mask, dtype=bool)
     n0, n1 = m.shape
@@ -125,18 +126,24 @@ def grad_matrices(mask: np.ndarray):
         r += [me[both], me[both]]
         c += [ai[ii[both] + di, jj[both] + dj], ai[ii[both] - di, jj[both] - dj]]
         v += [np.full(both.sum(), 0.5), np.full(both.sum(), -0.5)]
-        fwd = has_p & ~has_m
-        if np.any(fwd & ~has_pp):
-            raise ValueError("one-sided difference needs two neighbours")
+        # one-sided second order where two cells exist on the open side; first order
+        # (y1 - y0) where only one does (never the case on the reference's 256^2 grid)
+        fwd = has_p & ~has_m & has_pp
         r += [me[fwd]] * 3
         c += [me[fwd], ai[ii[fwd] + di, jj[fwd] + dj], ai[ii[fwd] + 2 * di, jj[fwd] + 2 * dj]]
         v += [np.full(fwd.sum(), -1.5), np.full(fwd.sum(), 2.0), np.full(fwd.sum(), -0.5)]
-        bwd = has_m & ~has_p
-        if np.any(bwd & ~has_mm):
-            raise ValueError("one-sided difference needs two neighbours")
+        fwd1 = has_p & ~has_m & ~has_pp
+        r += [me[fwd1]] * 2
+        c += [me[fwd1], ai[ii[fwd1] + di, jj[fwd1] + dj]]
+        v += [np.full(fwd1.sum(), -1.0), np.full(fwd1.sum(), 1.0)]
+        bwd = has_m & ~has_p & has_mm
         r += [me[bwd]] * 3
         c += [me[bwd], ai[ii[bwd] - di, jj[bwd] - dj], ai[ii[bwd] - 2 * di, jj[bwd] - 2 * dj]]
         v += [np.full(bwd.sum(), 1.5), np.full(bwd.sum(), -2.0), np.full(bwd.sum(), 0.5)]
+        bwd1 = has_m & ~has_p & ~has_mm
+        r += [me[bwd1]] * 2
+        c += [me[bwd1], ai[ii[bwd1] - di, jj[bwd1] - dj]]
+        v += [np.full(bwd1.sum(), 1.0), np.full(bwd1.sum(), -1.0)]
         mats.append(sp.csr_matrix((np.concatenate(v), (np.concatenate(r), np.concatenate(c))), shape=(n, n)))
     return mats
 

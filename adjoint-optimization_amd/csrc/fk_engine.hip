@@ -911,6 +911,40 @@ int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every
     return FKADJ_OK;
 }
 
+int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *const *E, int nsteps)
+{
+    if (count < 1 || !insts || !E || nsteps < 0) return FKADJ_ERR_ARG;
+    for (int k = 0; k < count; k++) {
+        fkadj_instance *I = insts[k];
+        if (!I || !E[k] || I->prec != FKADJ_F32 || I->nt != insts[0]->nt || I->K != insts[0]->K || nsteps > I->nt)
+            return FKADJ_ERR_ARG;
+        for (int q = 0; q < k; q++)
+            if (insts[q] == I) return FKADJ_ERR_ARG;
+        if (!I->state_set) return FKADJ_ERR_STATE;
+        if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    }
+    hipStream_t s = insts[0]->stream;
+    for (int k = 0; k < count; k++) {
+        CHK(build_tables(insts[k], E[k]));
+        HIPCHK(hipStreamSynchronize(insts[k]->stream));
+    }
+    std::vector<fkadj_instance *> members(insts, insts + count);
+    FusedGroupDev *g = nullptr;
+    CHK(group_build(members, s, &g));
+    int rc = FKADJ_OK;
+    for (auto *I : members)
+        if (hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            rc = FKADJ_ERR_DEVICE;
+    for (int i = 0; i < nsteps && !rc; i++) group_forward(g, MODE_FWD, i, 0, s);
+    if (!rc && hipGetLastError() != hipSuccess) rc = FKADJ_ERR_DEVICE;
+    for (auto *I : members)
+        if (!rc && hipMemcpyAsync(I->d_y0, fwd_loc(I, nsteps), I->st_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+            rc = FKADJ_ERR_DEVICE;
+    if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = FKADJ_ERR_DEVICE;
+    group_free(g);
+    return rc;
+}
+
 int fkadj_stimulate(fkadj_instance *I, const uint8_t *cell_mask, double value)
 {
     if (!I || !cell_mask || !std::isfinite(value)) return FKADJ_ERR_ARG;

@@ -85,6 +85,7 @@ def lib():
     L.fkadj_forward.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
     L.fkadj_stimulate.argtypes = [vp, C.POINTER(C.c_uint8), C.c_double]
     L.fkadj_batch_grad.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
+    L.fkadj_batch_forward.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.c_int]
     L.fkadj_set_profiling.argtypes = [C.c_int]
     L.fkadj_last_timing.argtypes = [C.POINTER(Timing)]
     _lib = L

@@ -161,5 +161,14 @@ def batch_grad(instances, Es):
     return [(grads[k], Lc[k], L0[k], LE[k]) for k in range(n)]
 
 
+def batch_forward(instances, Es, nsteps):
+    """Advance the resident states of several FP32 instances by nsteps in shared launches."""
+    n = len(instances)
+    Es = [L._f64(np.ravel(E)) for E in Es]
+    hs = (C.c_void_p * n)(*[inst.h for inst in instances])
+    ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
+    check(L.lib().fkadj_batch_forward(n, hs, ep, int(nsteps)), "fkadj_batch_forward")
+
+
 def footprint_bytes(problem: L.Problem) -> int:
     return int(L.lib().fkadj_footprint_bytes(C.byref(problem)))

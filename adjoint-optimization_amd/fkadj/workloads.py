@@ -1,0 +1,96 @@
+"""Seeded synthetic workloads for the benchmark configurations.
+
+Shared physics: the reference's FK constants (src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-21)
+with h = 0.025 cm, dt = 0.02 ms, Du = 1e-3 cm^2/ms (Dv = Dw = 1e-5), no-flux walls and disc holes
+of radius 2-6 cells.  Pulse: K piecewise-constant amplitudes A_k ~ U[0, 1] V/cm, passed to the
+model in dV/cm (x10, the reference's unit, src/scripting/python/smooth_biphasic_pulse.py:18).
+Objective: mean over active cells of u(T)^2 + lambda * integral E^2 dt with lambda = 1e-3, i.e.
+S = (2/n_active) I, gamma = (1, 0, 0), alpha = 2 lambda in the reference's L0/2 + alpha LE/2.
+
+Initial states: an S1-S2 cross-field protocol.  S1 excites a strip at the left wall; the planar
+wave (about 1.2 cells/ms, action potential about 180 ms at these constants) crosses the tissue.
+When its back has passed mid-width, S2 excites the lower half behind the back; the S2 wave's
+free end at the half-height line curls into a spiral.  The S2 time is jittered from the seed.
+"""
+from __future__ import annotations
+
+import numpy as np
+
+from . import _lib as L
+from . import geometry as G
+from .api import Instance, batch_forward
+
+H_CM = 0.025
+DT_MS = 0.02
+LAMBDA = 1e-3
+DV_PER_V = 10.0          # model field unit dV/cm per V/cm
+CV_CELLS_PER_MS = 1.2    # planar conduction velocity at h = 0.025 (measured with the oracle)
+APD_MS = 180.0           # action potential duration (u > 0.5)
+
+CONFIGS = {
+    "C1": dict(N=128, holes=20, T=100.0, K=100, precision=L.F64),
+    "C2": dict(N=512, holes=80, T=200.0, K=200, precision=L.F32, ckpt=50),
+    "C3": dict(sizes=(256, 384, 512, 768, 1024), count=64, T=150.0, K=150, precision=L.F32, ckpt=50),
+    "C5": dict(N=2048, holes=400, T=500.0, K=500, precision=L.F32, ckpt=50),
+}
+
+
+def model():
+    return L.reference_model(h=H_CM)
+
+
+def holes_mask(N: int, n_holes: int, seed: int, ncols: int | None = None) -> np.ndarray:
+    return G.disc_holes(N, ncols or N, n_holes, 2, 6, seed)
+
+
+def amplitudes(K: int, seed: int) -> np.ndarray:
+    """K amplitudes in the model unit (dV/cm)."""
+    return DV_PER_V * np.random.default_rng(1000 + seed).uniform(0.0, 1.0, K)
+
+
+def rest_state(n_act: int) -> np.ndarray:
+    return np.concatenate([np.zeros(n_act), np.ones(n_act), np.ones(n_act)])
+
+
+def make_instance(mask: np.ndarray, T: float, K: int, precision=L.F32, ckpt=50, tile_rows=0) -> Instance:
+    nt = int(round(T / DT_MS))
+    n_act = int(mask.sum())
+    return Instance(G.active_index_from_mask(mask), mask.shape[0], mask.shape[1], dt=DT_MS, nt=nt, model=model(),
+                    pulse_mode=L.PULSE_PIECEWISE, n_amp=K, alpha=2 * LAMBDA, gamma=(1.0, 0.0, 0.0),
+                    s_diag=2.0 / n_act, precision=precision, ckpt=ckpt, tile_rows=tile_rows)
+
+
+def spiral_states(masks, seeds, chunk_ms: float = 5.0, settle_ms: float = 150.0):
+    """S1-S2 spiral initial states (packed) for grids of one shape, prepared in shared launches."""
+    nrows, ncols = masks[0].shape
+    chunk = int(round(chunk_ms / DT_MS))
+    prep = []
+    for m in masks:
+        inst = Instance(G.active_index_from_mask(m), nrows, ncols, dt=DT_MS, nt=chunk, model=model(),
+                        pulse_mode=L.PULSE_PIECEWISE, n_amp=1, alpha=0.0, gamma=(1.0, 0.0, 0.0),
+                        s_diag=1.0, precision=L.F32, ckpt=chunk)
+        inst.set_state(rest_state(int(m.sum())))
+        s1 = np.zeros((nrows, ncols), bool)
+        s1[:, :6] = True
+        inst.stimulate(s1, 1.0)
+        prep.append(inst)
+    t_back = APD_MS + 0.5 * ncols / CV_CELLS_PER_MS
+    t_s2 = [t_back + np.random.default_rng(seed).uniform(-10.0, 10.0) for seed in seeds]
+    back_col = [int(0.5 * ncols - 12) for _ in seeds]
+    done = [False] * len(masks)
+    t = 0.0
+    t_end = max(t_s2) + settle_ms
+    zero = [np.zeros(1)] * len(masks)
+    while t < t_end - 1e-9:
+        batch_forward(prep, zero, chunk)
+        t += chunk_ms
+        for k, inst in enumerate(prep):
+            if not done[k] and t >= t_s2[k]:
+                s2 = np.zeros((nrows, ncols), bool)
+                s2[nrows // 2:, :back_col[k]] = True
+                inst.stimulate(s2, 1.0)
+                done[k] = True
+    states = [inst.get_state() for inst in prep]
+    for inst in prep:
+        inst.close()
+    return states

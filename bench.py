@@ -1,0 +1,201 @@
+"""Benchmark: gradient evaluations per second (forward + adjoint) of the Fenton-Karma pulse
+optimiser at 512x512, FP32 state with FP64 reductions, batched on one MI355X, beside the plain
+single-instance FP64 path, with the FP32 error against FP64 and the dominant kernel's roofline.
+
+Workload (per instance): 512x512 grid, 80 disc holes (radius 2-6 cells), h = 0.025 cm,
+dt = 0.02 ms, T = 200 ms (10 000 steps), 200 piecewise-constant amplitudes, checkpoints every 50
+steps with recomputation, objective mean(u(T)^2) + 1e-3 * pulse energy, S1-S2 spiral initial
+state; every instance has its own seed.  A step is one batched gradient evaluation of all
+instances.  Inputs are resident in HBM before the timed region.
+
+Prints one JSON line on stdout; progress goes to stderr.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
+
+HBM_PEAK_GBS = 8000.0
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def algorithmic_bytes_per_cell_step(K: int, w: int = 4) -> float:
+    """Compulsory traffic of a single-step-fused evaluation: forward/recompute 7 words (read u, v, w,
+    gh; write u, v, w), adjoint 13 words (read z, y_i, y_{i-1}, gh; write z); R = (K-1)/K recompute."""
+    R = (K - 1) / K
+    return w * (7 * (1 + R) + 13)
+
+
+def cpu_baseline(mask, y0, A, nt_full, nt_sample=100):
+    """FP64 oracle (C, OpenMP) on the host: a bounded sample of the same workload (the first
+    nt_sample steps of instance 0, all three sweeps), scaled to a full evaluation."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fkadj import geometry as G
+    from fkadj import workloads as W
+
+    om = O.reference_model(h=W.H_CM)
+    ai = G.active_index_from_mask(mask)
+    planes = O.num2nan(y0, ai, *mask.shape)
+    gh = np.full(mask.size, np.nan)
+    gh[ai > 0] = om.Du * 2 / om.h * G.ghost_lr_from_mask(mask)
+    P = nt_full // len(A)
+    Ef = np.repeat(A, 3 * P)[: 3 * nt_sample]
+    S = G.mean_u2_S(int(mask.sum()))
+    t0 = time.perf_counter()
+    O.grad(om, planes, gh.reshape(mask.shape), W.DT_MS, Ef, Ef.copy(), ai, G.csc_arrays(S), (1, 0, 0), ckpt=50)
+    dt = time.perf_counter() - t0
+    per_eval = dt * nt_full / nt_sample
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    return {"value": 1.0 / per_eval, "unit": "evals/s", "cores": threads, "kind": "port",
+            "sample": f"FP64 oracle, instance 0 (512^2), first {nt_sample} of {nt_full} steps with forward, "
+                      f"recompute and adjoint sweeps ({dt:.2f} s), scaled x{nt_full // nt_sample}",
+            "seconds_per_eval": per_eval}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--grid", type=int, default=512)
+    ap.add_argument("--T", type=float, default=200.0)
+    ap.add_argument("--skip-plain", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--latency-evals", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched instances of this "
+              "workload are independent and would shard by instance with no communication.", file=sys.stderr)
+        sys.exit(2)
+
+    import fkadj
+    from fkadj import workloads as W
+
+    cfg = W.CONFIGS["C2"]
+    N, holes, K_amp, ckpt = args.grid, cfg["holes"], cfg["K"], cfg["ckpt"]
+    T = args.T
+    nt = int(round(T / W.DT_MS))
+    B = args.batch
+    seeds = list(range(B))
+    t0 = time.perf_counter()
+    masks = [W.holes_mask(N, holes, s) for s in seeds]
+    log(f"[bench] preparing {B} spiral states at {N}^2")
+    states = W.spiral_states(masks, seeds)
+    log(f"[bench] states ready in {time.perf_counter() - t0:.1f} s")
+    A = [W.amplitudes(K_amp, s) for s in seeds]
+    insts = []
+    for m, y in zip(masks, states):
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt)
+        inst.set_state(y)
+        insts.append(inst)
+
+    # warmup + timed batched evaluations (HIP-event kernel sampling on during the timed region)
+    for w in range(args.warmup):
+        fkadj.batch_grad(insts, A)
+        log(f"[bench] warmup {w + 1}/{args.warmup} done")
+    fkadj.set_profiling(True)
+    step_ms, timings = [], []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        res = fkadj.batch_grad(insts, A)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        timings.append(fkadj.last_timing())
+        log(f"[bench] step {k + 1}/{args.steps}: {step_ms[-1]:.1f} ms")
+    total = time.perf_counter() - t_start
+    fkadj.set_profiling(False)
+    value = B * args.steps / total
+
+    # single-instance FP32 latency
+    lat = []
+    for k in range(args.latency_evals):
+        ts = time.perf_counter()
+        g32_single = insts[0].grad(A[0])
+        lat.append(1e3 * (time.perf_counter() - ts))
+    g_alone = g32_single[0]
+    bit_identical = bool(np.array_equal(g_alone, res[0][0]) and g32_single[1] == res[0][1])
+    log(f"[bench] single FP32 eval {np.median(lat):.1f} ms; alone == batched: {bit_identical}")
+
+    # plain FP64 single instance: the baseline path and the FP64 reference of instance 0
+    plain = None
+    rel = None
+    if not args.skip_plain:
+        inst64 = W.make_instance(masks[0], T, K_amp, precision=fkadj.F64, ckpt=ckpt)
+        inst64.set_state(states[0])
+        ts = time.perf_counter()
+        g64, Lc64, L064, LE64 = inst64.grad(A[0])
+        t64 = time.perf_counter() - ts
+        inst64.close()
+        plain = {"value": 1.0 / t64, "unit": "evals/s", "ms_per_eval": 1e3 * t64,
+                 "path": "FP64, one kernel per RK stage, single instance"}
+        g32, Lc32 = res[0][0], res[0][1]
+        rel = {"grad_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+               "grad_rel_max": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+               "J_rel": float(abs(Lc32 - Lc64) / abs(Lc64)), "J_f64": Lc64, "J_f32": Lc32}
+        log(f"[bench] plain FP64 {1e3 * t64:.0f} ms/eval; rel err grad {rel['grad_rel_l2']:.2e} J {rel['J_rel']:.2e}")
+
+    # roofline of the dominant kernel from the in-run HIP-event samples
+    cells = sum(inst.nrows * ((inst.ncols + 63) // 64 * 64) for inst in insts)
+    fwd_ms = float(np.mean([t["fwd_kernel_ms"] for t in timings]))
+    adj_ms = float(np.mean([t["adj_kernel_ms"] for t in timings]))
+    nf = timings[-1]["fwd_launches"]
+    na = timings[-1]["adj_launches"]
+    fwd_bytes = 7 * 4 * cells
+    adj_bytes = 13 * 4 * cells
+    fwd_share, adj_share = fwd_ms * nf, adj_ms * na
+    if adj_share >= fwd_share:
+        dom, dbytes, dms = "k_fused_adj<16> (adjoint step)", adj_bytes, adj_ms
+    else:
+        dom, dbytes, dms = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
+    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "kernel_ms": dms, "bytes_per_launch": dbytes,
+            "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
+            "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},
+            "eval_GBs": eval_bytes * args.steps / total / 1e9}
+
+    cpu = None if args.skip_cpu else cpu_baseline(masks[0], states[0], A[0], nt)
+    if cpu:
+        log(f"[bench] cpu baseline {cpu['seconds_per_eval']:.1f} s/eval on {cpu['cores']} threads")
+
+    out = {
+        "metric": "gradient evals/s (fwd+adjoint, 512^2, rel err<=1e-4) batched vs plain; p99 ms",
+        "value": value, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": float(np.mean(step_ms)), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "config": {"workload": f"C2 x{B} batched: {N}^2 FP32, {holes} holes, T={T:g} ms, dt={W.DT_MS} ms, "
+                               f"{K_amp} amplitudes, checkpoint every {ckpt} steps, S1-S2 spirals",
+                   "grid": N, "batch": B, "nt": nt, "n_amp": K_amp, "ckpt": ckpt},
+        "plain_f64": plain,
+        "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "rel_err_vs_f64": rel,
+        "alone_equals_batched": bit_identical,
+        "latency_ms": {"batched_step_p50": float(np.percentile(step_ms, 50)),
+                       "batched_step_p99": float(np.percentile(step_ms, 99)),
+                       "single_f32_p50": float(np.percentile(lat, 50)),
+                       "single_f32_p99": float(np.percentile(lat, 99))},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    for inst in insts:
+        inst.close()
+
+
+if __name__ == "__main__":
+    main()

@@ -141,6 +141,10 @@ int fkadj_stimulate(fkadj_instance *inst, const uint8_t *cell_mask, double value
 int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
                      double *Lcost, double *L0, double *LE);
 
+/* Forward solve of `count` FP32 instances (equal nt and ckpt) over nsteps in shared launches;
+ * each final state replaces the instance's resident state (used to prepare initial states). */
+int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *const *E, int nsteps);
+
 /* ---------------------------------------------------------------- timing */
 typedef struct fkadj_timing {
     double eval_ms;          /* last gradient evaluation, device time between HIP events */

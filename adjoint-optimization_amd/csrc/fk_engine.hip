@@ -121,6 +121,7 @@ struct fkadj_instance {
     double *d_val = nullptr;
     double *d_gamma = nullptr;
     char *d_y0 = nullptr, *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
+    char *d_kseg = nullptr, *d_kT = nullptr;   // FP32: stored slopes of the recompute sweep
     double *d_Ef = nullptr, *d_Ea = nullptr;
     double *d_part = nullptr, *d_partT = nullptr, *d_zgh = nullptr, *d_L0part = nullptr, *d_L0 = nullptr;
     double *d_packed = nullptr;
@@ -130,7 +131,7 @@ struct fkadj_instance {
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
 
     // fused path
-    int ty_f = 16;
+    int fwd_cfg = TILE_16_256, adj_cfg = TILE_16_256;   // fused tile configurations
     FusedSlot slot_h{};
     struct FusedGroupDev *group = nullptr;   // this instance alone, for fkadj_grad / fkadj_forward
 
@@ -144,8 +145,9 @@ struct fkadj_instance {
 struct FusedGroupDev {
     std::vector<fkadj_instance *> members;
     FusedSlot *d_slots = nullptr;
-    TileTask *d_f16 = nullptr, *d_f32 = nullptr, *d_adj = nullptr;
-    int n16 = 0, n32 = 0, nadj = 0;
+    // one tile list per kernel configuration (index = tile config id)
+    TileTask *d_fwd[4] = {nullptr, nullptr, nullptr, nullptr}, *d_adj[4] = {nullptr, nullptr, nullptr, nullptr};
+    int nfwd[4] = {0, 0, 0, 0}, nadj[4] = {0, 0, 0, 0};
 };
 
 namespace {
@@ -190,12 +192,18 @@ void fill_models(fkadj_instance *I)
     b.ghc = (float)a.ghc;
 }
 
-// Fused tile height per grid: 32-row forward tiles halve the halo recomputation but give half
-// as many workgroups; small grids keep 16 rows so that one instance still fills the GPU.
-int autotune_tile_rows(int rows, int pitch)
+// Built-in tile table.  Large tiles cut the halo recomputation (x1.25 at 64x16, x1.15 at 64x32 for
+// the forward) but give fewer workgroups; a grid keeps the smaller tile unless the larger one still
+// yields at least two workgroups per CU.
+int default_fwd_cfg(int rows, int pitch)
 {
     long tiles32 = (long)(pitch / FUSED_TX) * ((rows + 31) / 32);
-    return tiles32 >= 512 ? 32 : 16;
+    return tiles32 >= 512 ? TILE_32_512 : TILE_16_256;
+}
+int default_adj_cfg(int rows, int pitch)
+{
+    long tiles32 = (long)(pitch / FUSED_TX) * ((rows + 31) / 32);
+    return tiles32 >= 512 ? TILE_32_512 : TILE_16_256;
 }
 
 // XCD-aware order: the GPU deals workgroups round-robin over its 8 XCDs, each with its own L2.
@@ -489,9 +497,10 @@ void group_free(FusedGroupDev *g)
 {
     if (!g) return;
     if (g->d_slots) (void)hipFree(g->d_slots);
-    if (g->d_f16) (void)hipFree(g->d_f16);
-    if (g->d_f32) (void)hipFree(g->d_f32);
-    if (g->d_adj) (void)hipFree(g->d_adj);
+    for (int c = 0; c < 4; c++) {
+        if (g->d_fwd[c]) (void)hipFree(g->d_fwd[c]);
+        if (g->d_adj[c]) (void)hipFree(g->d_adj[c]);
+    }
     delete g;
 }
 
@@ -501,40 +510,43 @@ int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, Fus
     if (!g) return FKADJ_ERR_NOMEM;
     g->members = members;
     std::vector<FusedSlot> slots;
-    std::vector<TileTask> f16, f32, adj;
+    std::vector<TileTask> fl[4], al[4];
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k];
         slots.push_back(I->slot_h);
-        auto ft = make_tiles((int)k, I->rows, I->pitch, I->ty_f);
-        (I->ty_f == 32 ? f32 : f16).insert((I->ty_f == 32 ? f32 : f16).end(), ft.begin(), ft.end());
-        auto at = make_tiles((int)k, I->rows, I->pitch, 16);
-        adj.insert(adj.end(), at.begin(), at.end());
+        auto ft = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(I->fwd_cfg));
+        fl[I->fwd_cfg].insert(fl[I->fwd_cfg].end(), ft.begin(), ft.end());
+        auto at = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(I->adj_cfg));
+        al[I->adj_cfg].insert(al[I->adj_cfg].end(), at.begin(), at.end());
     }
-    f16 = xcd_order(f16);
-    f32 = xcd_order(f32);
-    adj = xcd_order(adj);
-    g->n16 = (int)f16.size();
-    g->n32 = (int)f32.size();
-    g->nadj = (int)adj.size();
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
-              hipMalloc((void **)&g->d_adj, sizeof(TileTask) * adj.size()) == hipSuccess &&
-              (f16.empty() || hipMalloc((void **)&g->d_f16, sizeof(TileTask) * f16.size()) == hipSuccess) &&
-              (f32.empty() || hipMalloc((void **)&g->d_f32, sizeof(TileTask) * f32.size()) == hipSuccess);
+              hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+    for (int c = 0; c < 4 && ok; c++) {
+        for (int side = 0; side < 2 && ok; side++) {
+            std::vector<TileTask> v = xcd_order(side == 0 ? fl[c] : al[c]);
+            if (v.empty()) continue;
+            TileTask *&d = side == 0 ? g->d_fwd[c] : g->d_adj[c];
+            (side == 0 ? g->nfwd[c] : g->nadj[c]) = (int)v.size();
+            ok = hipMalloc((void **)&d, sizeof(TileTask) * v.size()) == hipSuccess &&
+                 hipMemcpyAsync(d, v.data(), sizeof(TileTask) * v.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+        }
+    }
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) { group_free(g); return FKADJ_ERR_NOMEM; }
-    ok = hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-         hipMemcpyAsync(g->d_adj, adj.data(), sizeof(TileTask) * adj.size(), hipMemcpyHostToDevice, s) == hipSuccess &&
-         (f16.empty() || hipMemcpyAsync(g->d_f16, f16.data(), sizeof(TileTask) * f16.size(), hipMemcpyHostToDevice, s) == hipSuccess) &&
-         (f32.empty() || hipMemcpyAsync(g->d_f32, f32.data(), sizeof(TileTask) * f32.size(), hipMemcpyHostToDevice, s) == hipSuccess) &&
-         hipStreamSynchronize(s) == hipSuccess;
-    if (!ok) { group_free(g); return FKADJ_ERR_DEVICE; }
     *out = g;
     return FKADJ_OK;
 }
 
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
-    if (g->n16) fused_forward_launch(16, g->d_slots, g->d_f16, g->n16, mode, step, seg_a, s);
-    if (g->n32) fused_forward_launch(32, g->d_slots, g->d_f32, g->n32, mode, step, seg_a, s);
+    for (int c = 1; c < 4; c++)
+        if (g->nfwd[c]) fused_forward_launch(c, g->d_slots, g->d_fwd[c], g->nfwd[c], mode, step, seg_a, s);
+}
+
+void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
+{
+    for (int c = 1; c < 4; c++)
+        if (g->nadj[c]) fused_adjoint_launch(c, g->d_slots, g->d_adj[c], g->nadj[c], i, seg_a, s);
 }
 
 int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
@@ -546,12 +558,14 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
     HIPCHK(hipGetLastError());
     for (auto *I : g->members) terminal<float>(I, s);
+    group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt);
-        for (int j = 0; j + 1 < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
+        // all b-a steps are replayed: the last one only to store the slope f(y_{b-1})
+        for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
         for (int i = b; i > a; i--)
-            smp.adj_launch(s, [&] { fused_adjoint_launch(16, g->d_slots, g->d_adj, g->nadj, i, a, s); });
+            smp.adj_launch(s, [&] { group_adjoint(g, i, a, s); });
         HIPCHK(hipGetLastError());
     }
     for (auto *I : g->members) sum_rows_launch(I->d_part, I->nb_part, nt, I->d_zgh, s);
@@ -668,7 +682,7 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t esz = p->precision == FKADJ_F32 ? 4 : 8;
     const int64_t st = 3 * n * esz;
     const int64_t nck = (p->nt + K - 1) / K + 1;
-    int64_t states = 1 + nck + (K + 1) + 2 + 2 + (p->precision == FKADJ_F64 ? 6 : 0);
+    int64_t states = 1 + nck + (K + 1) + 2 + 2 + (p->precision == FKADJ_F64 ? 6 : 2 * K + 1);
     int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : (pitch / 64) * ((p->nrows + 15) / 16);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
                     8 * 3 * n;
@@ -681,6 +695,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     *out = nullptr;
     if (p->nrows < 4 || p->ncols < 4 || p->nt < 1 || !(p->dt > 0) || !std::isfinite(p->dt)) return FKADJ_ERR_ARG;
     if (p->precision != FKADJ_F64 && p->precision != FKADJ_F32) return FKADJ_ERR_ARG;
+    if (p->fwd_tile < 0 || p->fwd_tile > 3 || (p->adj_tile != 0 && p->adj_tile != TILE_16_256 && p->adj_tile != TILE_32_512))
+        return FKADJ_ERR_ARG;
     if (p->pulse_mode == FKADJ_PULSE_KNOTS) {
         if (p->nt + 1 < 4) return FKADJ_ERR_ARG;
     } else if (p->pulse_mode == FKADJ_PULSE_PIECEWISE) {
@@ -762,8 +778,11 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     const int nt = I->nt;
     I->nb_plain = (I->pitch / PLAIN_BX) * ((I->rows + PLAIN_BY - 1) / PLAIN_BY);
     if (I->prec == FKADJ_F32) {
-        I->ty_f = p->tile_rows == 32 || p->tile_rows == 16 ? p->tile_rows : autotune_tile_rows(I->rows, I->pitch);
-        I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + 15) / 16);   // adjoint tiles (16 rows)
+        I->fwd_cfg = (p->fwd_tile >= 1 && p->fwd_tile <= 3) ? p->fwd_tile : default_fwd_cfg(I->rows, I->pitch);
+        I->adj_cfg = (p->adj_tile == TILE_16_256 || p->adj_tile == TILE_32_512) ? p->adj_tile
+                                                                                 : default_adj_cfg(I->rows, I->pitch);
+        const int ty_a = tile_rows_of(I->adj_cfg);
+        I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + ty_a - 1) / ty_a);   // adjoint tiles
     } else {
         I->nb_part = I->nb_plain;
     }
@@ -776,6 +795,10 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     DMALLOC(I->d_ring, st * 2);
     DMALLOC(I->d_z, st * 2);
     if (I->prec == FKADJ_F64) DMALLOC(I->d_work, st * 6);
+    if (I->prec == FKADJ_F32) {
+        DMALLOC(I->d_kseg, st * 2 * I->K);
+        DMALLOC(I->d_kT, st);
+    }
     DMALLOC(I->d_Ef, sizeof(double) * 3 * nt);
     DMALLOC(I->d_Ea, sizeof(double) * 3 * nt);
     DMALLOC(I->d_part, sizeof(double) * (size_t)nt * I->nb_part);
@@ -811,6 +834,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
         S.ntiles = I->nb_part;
         S.code = I->d_code;
         S.ck = as<float>(I->d_ck); S.seg = as<float>(I->d_seg); S.ring = as<float>(I->d_ring); S.zb = as<float>(I->d_z);
+        S.kseg = as<float>(I->d_kseg); S.kT = as<float>(I->d_kT);
         S.Ef = I->d_Ef; S.Ea = I->d_Ea; S.partial = I->d_part;
         S.m = I->m32; S.dt = (float)p->dt;
         std::vector<fkadj_instance *> self{I};
@@ -828,7 +852,7 @@ int fkadj_destroy(fkadj_instance *I)
     if (!I) return FKADJ_ERR_ARG;
     if (I->stream) (void)hipStreamSynchronize(I->stream);
     dfree(I->d_code); dfree(I->d_cell_of); dfree(I->d_rowptr); dfree(I->d_colind); dfree(I->d_val); dfree(I->d_gamma);
-    dfree(I->d_y0); dfree(I->d_ck); dfree(I->d_seg); dfree(I->d_ring); dfree(I->d_z); dfree(I->d_work);
+    dfree(I->d_kseg); dfree(I->d_kT); dfree(I->d_y0); dfree(I->d_ck); dfree(I->d_seg); dfree(I->d_ring); dfree(I->d_z); dfree(I->d_work);
     dfree(I->d_Ef); dfree(I->d_Ea); dfree(I->d_part); dfree(I->d_partT); dfree(I->d_zgh); dfree(I->d_L0part);
     dfree(I->d_L0); dfree(I->d_packed); dfree(I->d_mask); group_free(I->group); I->group = nullptr;
     if (I->stream) (void)hipStreamDestroy(I->stream);

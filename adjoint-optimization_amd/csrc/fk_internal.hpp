@@ -15,18 +15,24 @@ struct GridArgs {
     const uint16_t *code;
 };
 
-// Deterministic FP64 sum over a 256-thread block (wavefront shuffles, then 4 partials in order).
-__device__ inline double block_sum_256(double v, int tid)
+// Deterministic FP64 sum over an NT-thread block (wavefront shuffles, then the NT/64 wavefront
+// partials added in order); the result is valid in thread 0.
+template <int NT>
+__device__ inline double block_sum(double v, int tid)
 {
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
-    __shared__ double part[4];
+    __shared__ double part[NT / 64];
     if ((tid & 63) == 0) part[tid >> 6] = v;
     __syncthreads();
     double s = 0.0;
-    if (tid == 0) s = ((part[0] + part[1]) + part[2]) + part[3];
+    if (tid == 0) {
+#pragma unroll
+        for (int w = 0; w < NT / 64; w++) s += part[w];
+    }
     return s;
 }
+__device__ inline double block_sum_256(double v, int tid) { return block_sum<256>(v, tid); }
 
 // ---- plain path context
 template <typename T, typename M>
@@ -49,8 +55,10 @@ void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
 
 // ---- fused FP32 path: one launch per time step over a tile work-list of one or more instances
 constexpr int FUSED_TX = 64;    // tile width: one wavefront row
-constexpr int FUSED_NT = 256;   // threads per workgroup
 constexpr int FUSED_HALO = 4;   // one cell per RK stage
+// tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512
+constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3;
+__host__ __device__ constexpr int tile_rows_of(int cfg) { return cfg == TILE_16_256 ? 16 : 32; }
 
 struct FusedSlot {
     int rows, pitch, n;
@@ -61,6 +69,8 @@ struct FusedSlot {
     float *seg;                 // K+1 states of the segment being recomputed (seg[0] unused)
     float *ring;                // 2 states for the forward sweep between checkpoints
     float *zb;                  // 2 adjoint states (ping-pong)
+    float *kseg;                // 2 x K slopes f(y_j) of recomputed steps (double-buffered by segment parity)
+    float *kT;                  // slope f(y_nt) at the final time with the adjoint's field value
     const double *Ef, *Ea;      // stage field tables, 3 per step
     double *partial;            // (nt) rows x ntiles: zgh partials of adjoint steps
     Model32 m;
@@ -71,16 +81,19 @@ struct TileTask {
     int slot, x0, y0, tile;     // instance, tile origin (cells), tile index within the instance
 };
 
-enum FusedMode { MODE_FWD = 0, MODE_REC = 1 };
+// MODE_FWD: forward sweep (checkpoints + ring); MODE_REC: segment recompute, also stores the
+// stage-1 slopes for the adjoint's Hermite reconstruction; MODE_RHS: only f(y_nt) -> kT.
+enum FusedMode { MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2 };
 
-// Launchers (stream-ordered, no synchronisation).
+// Launchers (stream-ordered, no synchronisation); cfg is a tile configuration.
 //   forward step `step` (0-based) of a sweep: MODE_FWD reads/writes ring/checkpoints,
 //   MODE_REC recomputes inside segment `seg_a` (first step of the segment) into seg[].
-void fused_forward_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          int mode, int step, int seg_a, hipStream_t stream);
-//   adjoint step over [t_{i-1}, t_i] for global i (seg_a = first step of the current segment).
-void fused_adjoint_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          int i, int seg_a, hipStream_t stream);
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
+                          int step, int seg_a, hipStream_t stream);
+//   adjoint step over [t_{i-1}, t_i] for global i (seg_a = first step of the current segment);
+//   cfg TILE_16_256 or TILE_32_512.
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
+                          int seg_a, hipStream_t stream);
 
 // ---- common small kernels (fk_kernels_common.hip)
 // z (terminal adjoint, T planes) and L0 partials from y(T): CSR S over packed indices or s_diag*I.

@@ -7,7 +7,7 @@
 // A 64 x TY output tile is staged in LDS with a 4-cell halo (one cell per RK stage); stage k is
 // evaluated on the tile grown by 4-k cells, so the tile needs no neighbour communication inside a
 // step and HBM sees one read of the inputs and one write of the outputs per step.
-// Work split: thread t owns the tile cells t + 256*s (one wavefront = one 64-cell row) and, in each
+// Work split: thread t owns the tile cells t + NT*s (one wavefront = one 64-cell row) and, in each
 // halo ring at distance d = 1..3 from the tile, the ring cell with perimeter index t.  The
 // ownership is the same in every stage, so per-cell values (state, RK increments, Jacobians) stay
 // in registers across the stages; LDS only holds the values neighbours read.
@@ -15,10 +15,11 @@
 
 namespace fk {
 
-template <int TY>
+template <int TY, int NT_>
 struct Tile {
-    static constexpr int TX = FUSED_TX, H = FUSED_HALO, NT = FUSED_NT;
+    static constexpr int TX = FUSED_TX, H = FUSED_HALO, NT = NT_;
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
+    static constexpr int RXP = RX + 1;               // odd LDS row pitch: column walks hit distinct banks
     static constexpr int TS = TX * TY / NT;          // tile cells per thread
     static constexpr int NSLOT = TS + 3;              // + one cell in each of the 3 halo rings
     static_assert(TX * TY % NT == 0, "tile must split evenly");
@@ -28,10 +29,10 @@ struct Tile {
 
 // Cell of slot `sl` of thread `tid` (LDS coordinates); returns false if the thread owns no
 // cell in that ring.
-template <int TY>
+template <class G>
 __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
 {
-    using G = Tile<TY>;
+    constexpr int TY = G::RY - 2 * G::H;
     if (sl < G::TS) {
         int lin = tid + G::NT * sl;
         ly = G::H + lin / G::TX;
@@ -53,8 +54,8 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
 }
 
 // Slot sl takes part in stage k (1..4): tile cells always, ring d while d <= 4-k.
-template <int TY>
-__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < Tile<TY>::TS || (sl - Tile<TY>::TS + 1) <= 4 - k; }
+template <class G>
+__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || (sl - G::TS + 1) <= 4 - k; }
 
 template <int RY, int RX>
 __device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], uint32_t c, int ly, int lx, float E,
@@ -94,12 +95,13 @@ __device__ inline void jtz(const Jac32 &J, const float *z, float *o)
     o[2] = J.a20 * z[0] + J.a22 * z[2];
 }
 
-template <int TY, int RY, int RX>
+template <class G>
 __device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
-                                   float (*dst)[RY][RX], uint16_t (*sc)[RX])
+                                   float (*dst)[G::RY][G::RXP], uint16_t (*sc)[G::RXP])
 {
+    constexpr int RX = G::RX, RY = G::RY;
     const int gx0 = x0 - FUSED_HALO, gy0 = y0 - FUSED_HALO;
-    for (int idx = tid; idx < RX * RY; idx += FUSED_NT) {
+    for (int idx = tid; idx < RX * RY; idx += G::NT) {
         int ly = idx / RX, lx = idx - ly * RX;
         int gy = gy0 + ly, gx = gx0 + lx;
         bool in = (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
@@ -111,13 +113,13 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 }
 
 // ------------------------------------------------------------------------------ forward
-template <int TY>
-__global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restrict__ slots,
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ slots,
                                                         const TileTask *__restrict__ tasks, int mode, int step,
                                                         int seg_a)
 {
-    using G = Tile<TY>;
-    constexpr int RX = G::RX, RY = G::RY, NS = G::NSLOT;
+    using G = Tile<TY, NT>;
+    constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT;
     __shared__ float sY[3][RY][RX];
     __shared__ uint16_t sC[RY][RX];
 
@@ -127,21 +129,32 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restr
     const size_t st = (size_t)3 * S.n;
     const float *yin;
     float *yout;
+    float *kout = nullptr;      // stage-1 slopes of the tile (MODE_REC, MODE_RHS)
+    float E1, E23, E4;
     if (mode == MODE_FWD) {
         yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
         int nx = step + 1;
         yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
                                              : S.ring + (size_t)(nx & 1) * st;
-    } else {
+    } else if (mode == MODE_REC) {
         int j = step - seg_a;
         yin = (j == 0) ? S.ck + (size_t)(seg_a / S.K) * st : S.seg + (size_t)j * st;
         yout = S.seg + (size_t)(j + 1) * st;
+        kout = S.kseg + ((size_t)((seg_a / S.K) & 1) * S.K + j) * st;
+    } else {
+        yin = S.ck + (size_t)(S.nck - 1) * st;
+        yout = nullptr;
+        kout = S.kT;
+    }
+    if (mode == MODE_RHS) {
+        E1 = E23 = E4 = (float)S.Ea[3 * (S.nt - 1) + 1];
+    } else {
+        E1 = (float)S.Ef[3 * step]; E23 = (float)S.Ef[3 * step + 1]; E4 = (float)S.Ef[3 * step + 2];
     }
     const Model32 m = S.m;
     const float dt = S.dt;
-    const float E1 = (float)S.Ef[3 * step], E23 = (float)S.Ef[3 * step + 1], E4 = (float)S.Ef[3 * step + 2];
 
-    load_region<TY, RY, RX>(S, yin, t.x0, t.y0, tid, sY, sC);
+    load_region<G>(S, yin, t.x0, t.y0, tid, sY, sC);
     __syncthreads();
 
     int ly[NS], lx[NS];
@@ -150,7 +163,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restr
     float y[NS][3], acc[G::TS][3], f[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        own[sl] = slot_cell<TY>(sl, tid, ly[sl], lx[sl]);
+        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
         cc[sl] = sC[ly[sl]][lx[sl]];
 #pragma unroll
@@ -162,12 +175,24 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restr
         const float E = (k == 1) ? E1 : (k == 4 ? E4 : E23);
 #pragma unroll
         for (int sl = 0; sl < NS; sl++) {
-            if (!slot_in_stage<TY>(sl, k)) continue;
+            if (!slot_in_stage<G>(sl, k)) continue;
             f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
             if (own[sl] && (cc[sl] & C_ACTIVE)) {
                 float yc[3];
                 rhs_lds<RY, RX>(m, sY, cc[sl], ly[sl], lx[sl], E, m.ghc * (float)code_gh(cc[sl]), f[sl], yc);
             }
+        }
+        if (k == 1 && kout) {
+            // stage-1 slope f(y_j) of the tile, read back by the adjoint step
+#pragma unroll
+            for (int sl = 0; sl < G::TS; sl++) {
+                int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+                if (gy >= S.rows) continue;
+                int kk = gy * S.pitch + gx;
+#pragma unroll
+                for (int v = 0; v < 3; v++) kout[v * S.n + kk] = f[sl][v];
+            }
+            if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
         }
         if (k < 4) {
             const float a = (k == 3) ? dt : 0.5f * dt;
@@ -175,7 +200,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_fwd(const FusedSlot *__restr
             __syncthreads();
 #pragma unroll
             for (int sl = 0; sl < NS; sl++) {
-                if (!slot_in_stage<TY>(sl, k) || !own[sl]) continue;
+                if (!slot_in_stage<G>(sl, k) || !own[sl]) continue;
 #pragma unroll
                 for (int v = 0; v < 3; v++) sY[v][ly[sl]][lx[sl]] = y[sl][v] + a * f[sl][v];
                 if (sl < G::TS) {
@@ -213,14 +238,14 @@ __device__ inline T h4p(T x0, T x1, T m0, T m1)
            T(-s * s * (6.0 - 16.0 * s + 9.0 * s * s)) * x1 + T((s - 1.0) * (9.0 * s - 5.0) * s * s / 4.0) * m1;
 }
 
-template <int TY>
-__global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restrict__ slots,
-                                                        const TileTask *__restrict__ tasks, int i, int seg_a)
+template <int TY, int NT>
+__global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ slots,
+                                                  const TileTask *__restrict__ tasks, int i, int seg_a)
 {
-    using G = Tile<TY>;
-    constexpr int RX = G::RX, RY = G::RY, NS = G::NSLOT, TS = G::TS;
-    __shared__ float sA[3][RY][RX];
-    __shared__ float sB[3][RY][RX];
+    using G = Tile<TY, NT>;
+    constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
+    __shared__ float sA[3][RY][RX];   // z, then the stage values Z_k
+    __shared__ float sB[3][RY][RX];   // y(t_{i-1} + dt/3)
     __shared__ uint16_t sC[RY][RX];
 
     const TileTask t = tasks[blockIdx.x];
@@ -229,60 +254,65 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
     const size_t st = (size_t)3 * S.n;
     const int segi = seg_a / S.K;
     const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
+    const float *kbuf = S.kseg + (size_t)(segi & 1) * S.K * st;
     const float *yA = (i - 1 == seg_a) ? S.ck + (size_t)segi * st : S.seg + (size_t)(i - 1 - seg_a) * st;
     const float *yB = (i == b_end) ? S.ck + (size_t)(segi + 1) * st : S.seg + (size_t)(i - seg_a) * st;
+    const float *kA = kbuf + (size_t)(i - 1 - seg_a) * st;
+    // f(y_i): slope of the next recomputed step, the first slope of the later segment, or f(y_nt)
+    const float *kB = (i < b_end) ? kbuf + (size_t)(i - seg_a) * st
+                                  : (b_end < S.nt ? S.kseg + (size_t)((segi + 1) & 1) * S.K * st : S.kT);
     const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
     float *zout = S.zb + (size_t)((S.nt - i + 1) & 1) * st;
     const Model32 m = S.m;
     const float dt = S.dt;
     const int row = i - 1;
-    const float EA = (float)S.Ea[3 * row], EB = (float)S.Ea[3 * row + 1], E3 = (float)S.Ea[3 * row + 2];
+    const float E3 = (float)S.Ea[3 * row + 2];
+    // the stored slope at y_i used the forward field of step i; the reconstruction needs the field
+    // of this interval (they differ at a piecewise-constant jump): f_u += (E_a - E_f) gh
+    const float dEB = (i < S.nt) ? (float)(S.Ea[3 * row + 1] - S.Ef[3 * i]) : 0.0f;
 
-    load_region<TY, RY, RX>(S, yA, t.x0, t.y0, tid, sA, sC);
-    load_region<TY, RY, RX>(S, yB, t.x0, t.y0, tid, sB, (uint16_t(*)[RX]) nullptr);
+    load_region<G>(S, zin, t.x0, t.y0, tid, sA, sC);
     __syncthreads();
 
-    int ly[NS], lx[NS];
+    int ly[NS], lx[NS], kk[NS];
     bool own[NS], act[NS];
     uint32_t cc[NS];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        own[sl] = slot_cell<TY>(sl, tid, ly[sl], lx[sl]);
+        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
         cc[sl] = sC[ly[sl]][lx[sl]];
         act[sl] = own[sl] && (cc[sl] & C_ACTIVE);
+        kk[sl] = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
     }
 
-    // phase 1a: end slopes, y(t_{i-1}+dt/3) and the partial midpoint value on R1
-    float y3[NS][3], y4[NS][3], yb[NS][3], ya[TS][3];
+    // phase 1a: y(t_{i-1}+dt/3) and the partial midpoint value on R1 from the stored end slopes
+    float y4[NS][3], yb[NS][3], ya[TS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
+        float y3[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (int v = 0; v < 3; v++) { y3[sl][v] = 0.0f; y4[sl][v] = 0.0f; yb[sl][v] = 0.0f; }
+        for (int v = 0; v < 3; v++) { y4[sl][v] = 0.0f; yb[sl][v] = 0.0f; }
         if (sl < TS) { ya[sl][0] = ya[sl][1] = ya[sl][2] = 0.0f; }
         if (act[sl]) {
+            const int q = kk[sl];
             const float gh = m.ghc * (float)code_gh(cc[sl]);
-            float fa[3], fb[3], a[3];
-            rhs_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], EA, gh, fa, a);
-            rhs_lds<RY, RX>(m, sB, cc[sl], ly[sl], lx[sl], EB, gh, fb, yb[sl]);
 #pragma unroll
             for (int v = 0; v < 3; v++) {
-                float m0 = fa[v] * dt, m1 = fb[v] * dt;
-                y3[sl][v] = h3<float>(a[v], yb[sl][v], m0, m1);
-                y4[sl][v] = h4p<float>(a[v], yb[sl][v], m0, m1);
-                if (sl < TS) ya[sl][v] = a[v];
+                float a = yA[v * S.n + q], b = yB[v * S.n + q];
+                float fb = kB[v * S.n + q];
+                if (v == 0) fb += dEB * gh;
+                float m0 = kA[v * S.n + q] * dt, m1 = fb * dt;
+                y3[v] = h3<float>(a, b, m0, m1);
+                y4[sl][v] = h4p<float>(a, b, m0, m1);
+                yb[sl][v] = b;
+                if (sl < TS) ya[sl][v] = a;
             }
         }
-    }
-    __syncthreads();
-    // y(1/3) -> sB (R1), z -> sA (whole region)
-#pragma unroll
-    for (int sl = 0; sl < NS; sl++)
         if (own[sl])
 #pragma unroll
-            for (int v = 0; v < 3; v++) sB[v][ly[sl]][lx[sl]] = y3[sl][v];
-    load_region<TY, RY, RX>(S, zin, t.x0, t.y0, tid, sA, (uint16_t(*)[RX]) nullptr);
-    __syncthreads();
+            for (int v = 0; v < 3; v++) sB[v][ly[sl]][lx[sl]] = y3[v];
+    }
 
     // phase 1b: adjoint stage 1 on R1 with J(y_i)
     float z[NS][3], Zk[NS][3], dz[TS][3];
@@ -318,7 +348,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
     Jac32 Jh[NS];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        if (!slot_in_stage<TY>(sl, 2)) continue;
+        if (!slot_in_stage<G>(sl, 2)) continue;
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
         if (!own[sl]) continue;
@@ -342,7 +372,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
     __syncthreads();
 #pragma unroll
     for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<TY>(sl, 2) && own[sl])
+        if (slot_in_stage<G>(sl, 2) && own[sl])
 #pragma unroll
             for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
@@ -350,7 +380,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
     // phase 3: adjoint stage 3 on R3 with J(yh)
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        if (!slot_in_stage<TY>(sl, 3)) continue;
+        if (!slot_in_stage<G>(sl, 3)) continue;
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
         if (!own[sl]) continue;
@@ -368,7 +398,7 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
     __syncthreads();
 #pragma unroll
     for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<TY>(sl, 3) && own[sl])
+        if (slot_in_stage<G>(sl, 3) && own[sl])
 #pragma unroll
             for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
@@ -386,34 +416,37 @@ __global__ __launch_bounds__(FUSED_NT) void k_fused_adj(const FusedSlot *__restr
 #pragma unroll
             for (int v = 0; v < 3; v++) zn[v] = z[sl][v] + (dz[sl][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
         }
-        int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+        int gy = t.y0 + ly[sl] - G::H;
         if (gy < S.rows) {
-            int kk = gy * S.pitch + gx;
 #pragma unroll
-            for (int v = 0; v < 3; v++) zout[v * S.n + kk] = zn[v];
+            for (int v = 0; v < 3; v++) zout[v * S.n + kk[sl]] = zn[v];
             int gh = code_gh(cc[sl]);
             if (act[sl] && gh != 0) part += (double)zn[0] * ((double)m.ghc * (double)gh);
         }
     }
-    double s = block_sum_256(part, tid);
+    double s = block_sum<NT>(part, tid);
     if (tid == 0) S.partial[(size_t)row * S.ntiles + t.tile] = s;
 }
 
 // ------------------------------------------------------------------------------ launchers
-void fused_forward_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          int mode, int step, int seg_a, hipStream_t stream)
+// Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512.
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
+                          int step, int seg_a, hipStream_t stream)
 {
-    if (tile_rows == 32)
-        hipLaunchKernelGGL(k_fused_fwd<32>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, mode, step, seg_a);
-    else
-        hipLaunchKernelGGL(k_fused_fwd<16>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, mode, step, seg_a);
+    switch (cfg) {
+    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    }
 }
 
-void fused_adjoint_launch(int tile_rows, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
                           int seg_a, hipStream_t stream)
 {
-    (void)tile_rows;  // the adjoint keeps 16-row tiles: its register footprint doubles with 32
-    hipLaunchKernelGGL(k_fused_adj<16>, dim3(ntasks), dim3(FUSED_NT), 0, stream, slots_dev, tasks_dev, i, seg_a);
+    switch (cfg) {
+    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
+    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
+    }
 }
 
 }  // namespace fk

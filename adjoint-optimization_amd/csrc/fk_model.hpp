@@ -93,29 +93,38 @@ __device__ inline void jacTz64(const Model64 &m, double u, double v, double w, d
 }
 
 // =====================================================================================
-// FP32: p = sigma(2 ksm (u-u_c)), q = sigma(2 ksm (u-u_v)), (1+tanh(k(u-u_csi)))/2 = sigma(2k(u-u_csi)).
-// One exp2 gives both p and q; 1-p = e_p * p avoids cancellation near p = 1.
+// FP32: p = sigma(2 ksm (u-u_c)), q = sigma(2 ksm (u-u_v)), (1+tanh(k(u-u_csi)))/2 = sigma(2k(u-u_csi)),
+// written with e_q = exp(-2 ksm (u-u_v)), e_p = e_q exp(2 ksm (u_c-u_v)), e_s = exp(-2k (u-u_csi)):
+//   p = 1/A, q = 1/B, sigma_s = 1/C with A = 1+e_p, B = 1+e_q, C = 1+e_s, and
+//   1/tau_v- = 1/(tau_v2 + q (tau_v1-tau_v2)) = B/Dv with Dv = tau_v2 B + (tau_v1-tau_v2).
+// One reciprocal r = 1/(A B C Dv) gives all four.  The exponents are capped at 2^24 so the
+// product stays below 2^127 (the gates then bottom out at 2^-24, FP32 rounding next to 1).
+// 1-p = e_p p (and alike) avoids the cancellation of 1 - p near p = 1.
 // =====================================================================================
 struct Gates32 {
-    float p, omp, q, omq, s, oms;   // p, 1-p, q, 1-q, sigma_si, 1-sigma_si
+    float p, omp, q, omq, s, oms, itvm;   // p, 1-p, q, 1-q, sigma_s, 1-sigma_s, 1/tau_v-
 };
 
 __device__ inline float fast_exp2(float x) { return __builtin_amdgcn_exp2f(x); }
 __device__ inline float fast_rcp(float x) { return __builtin_amdgcn_rcpf(x); }
 
-// The exponents are capped at 2^64 so that e_p = e_q * cpq stays finite (then p ~ 1e-27 and
-// 1-p = e_p p = 1 to FP32 precision, instead of inf * 0).
 __device__ inline Gates32 gates32(const Model32 &m, float u)
 {
     Gates32 g;
-    float eq = fast_exp2(fminf(fmaf(m.kq, u, m.bq), 64.0f));
+    float eq = fast_exp2(fminf(fmaf(m.kq, u, m.bq), 24.0f));
+    float es = fast_exp2(fminf(fmaf(m.ks, u, m.bs), 24.0f));
     float ep = eq * m.cpq;
-    float es = fast_exp2(fminf(fmaf(m.ks, u, m.bs), 64.0f));
-    g.q = fast_rcp(1.0f + eq);
-    g.omq = eq * g.q;
-    g.p = fast_rcp(1.0f + ep);
+    float A = 1.0f + ep, B = 1.0f + eq, C = 1.0f + es;
+    float Dv = fmaf(m.tau_v2m, B, m.dtv);
+    float AB = A * B, ABC = AB * C;
+    float r = fast_rcp(ABC * Dv);
+    float rD = r * Dv;                       // 1 / (A B C)
+    g.p = rD * (B * C);
+    g.q = rD * (A * C);
+    g.s = rD * AB;
+    g.itvm = r * (ABC * B);                  // B / Dv
     g.omp = ep * g.p;
-    g.s = fast_rcp(1.0f + es);
+    g.omq = eq * g.q;
     g.oms = es * g.s;
     return g;
 }
@@ -125,8 +134,7 @@ __device__ inline void fion32(const Model32 &m, const Gates32 &g, float u, float
 {
     float gg = (1.0f - u) * (u - m.u_c);
     fu = v * g.p * gg * m.i_tau_d - u * g.omp * m.i_tau_0 - g.p * m.i_tau_r + w * g.s * m.i_tau_si;
-    float itvm = fast_rcp(fmaf(g.q, m.dtv, m.tau_v2m));
-    fv = g.omp * (1.0f - v) * itvm - g.p * v * m.i_tau_vp;
+    fv = g.omp * (1.0f - v) * g.itvm - g.p * v * m.i_tau_vp;
     fw = g.omp * (1.0f - w) * m.i_tau_wm - g.p * w * m.i_tau_wp;
 }
 
@@ -143,7 +151,7 @@ __device__ inline Jac32 jac32(const Model32 &m, const Gates32 &g, float u, float
     float ds = m.two_k * g.s * g.oms;
     float gg = (1.0f - u) * (u - m.u_c);
     float dg = m.one_plus_uc - 2.0f * u;
-    float itvm = fast_rcp(fmaf(g.q, m.dtv, m.tau_v2m));
+    float itvm = g.itvm;
     J.a00 = v * (dp * gg + g.p * dg) * m.i_tau_d - (g.omp - u * dp) * m.i_tau_0 - dp * m.i_tau_r
           + w * ds * m.i_tau_si;
     J.a10 = g.p * gg * m.i_tau_d;

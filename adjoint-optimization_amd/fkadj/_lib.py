@@ -38,7 +38,7 @@ class Problem(C.Structure):
         ("alpha", C.c_double), ("gamma", C.c_double * 3),
         ("S_colptr", C.POINTER(C.c_int64)), ("S_rowind", C.POINTER(C.c_int64)), ("S_val", C.POINTER(C.c_double)),
         ("s_diag", C.c_double),
-        ("precision", C.c_int), ("ckpt", C.c_int), ("tile_rows", C.c_int),
+        ("precision", C.c_int), ("ckpt", C.c_int), ("fwd_tile", C.c_int), ("adj_tile", C.c_int),
     ]
 
 

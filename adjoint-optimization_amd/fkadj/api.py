@@ -58,7 +58,7 @@ class Instance:
 
     def __init__(self, active_index, nrows, ncols, *, dt, nt, model=None, ghost_lr=None, pulse_mode=L.PULSE_KNOTS,
                  n_amp=0, alpha=0.5, gamma=(1.0, 0.2, 0.2), S=None, s_diag=0.0, precision=L.F32, ckpt=50,
-                 tile_rows=0):
+                 fwd_tile=0, adj_tile=0):
         self._keep = []
         ai = np.ascontiguousarray(np.ravel(active_index), dtype=np.int32)
         p = L.Problem()
@@ -82,7 +82,7 @@ class Instance:
             p.S_rowind = rowind.ctypes.data_as(C.POINTER(C.c_int64))
             p.S_val = L._d(val)
         p.s_diag = float(s_diag)
-        p.precision, p.ckpt, p.tile_rows = int(precision), int(ckpt), int(tile_rows)
+        p.precision, p.ckpt, p.fwd_tile, p.adj_tile = int(precision), int(ckpt), int(fwd_tile), int(adj_tile)
         self.problem = p
         self.nrows, self.ncols, self.nt, self.dt = p.nrows, p.ncols, p.nt, p.dt
         self.precision = p.precision

@@ -100,7 +100,9 @@ typedef struct fkadj_problem {
     double s_diag;
     int precision;                    /* FKADJ_F64 / FKADJ_F32 */
     int ckpt;                         /* forward checkpoint interval K (steps); 0 -> 50 */
-    int tile_rows;                    /* FP32 fused tile height (16 or 32); 0 -> autotuned table */
+    int fwd_tile;                     /* FP32 forward tile: 1 = 64x16 / 256 threads, 2 = 64x32 / 256,
+                                         3 = 64x32 / 512; 0 -> built-in table by grid size */
+    int adj_tile;                     /* FP32 adjoint tile: 1 = 64x16 / 256, 3 = 64x32 / 512; 0 -> table */
 } fkadj_problem;
 
 typedef struct fkadj_instance fkadj_instance;

@@ -131,7 +131,7 @@ struct fkadj_instance {
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
 
     // fused path
-    int fwd_cfg = TILE_16_256, adj_cfg = TILE_16_256;   // fused tile configurations
+    int fwd_cfg = 0, adj_cfg = TILE_16_512;   // fused tile configurations (fwd 0: per group)
     FusedSlot slot_h{};
     struct FusedGroupDev *group = nullptr;   // this instance alone, for fkadj_grad / fkadj_forward
 
@@ -146,8 +146,8 @@ struct FusedGroupDev {
     std::vector<fkadj_instance *> members;
     FusedSlot *d_slots = nullptr;
     // one tile list per kernel configuration (index = tile config id)
-    TileTask *d_fwd[4] = {nullptr, nullptr, nullptr, nullptr}, *d_adj[4] = {nullptr, nullptr, nullptr, nullptr};
-    int nfwd[4] = {0, 0, 0, 0}, nadj[4] = {0, 0, 0, 0};
+    TileTask *d_fwd[NUM_TILE_CFG] = {}, *d_adj[NUM_TILE_CFG] = {};
+    int nfwd[NUM_TILE_CFG] = {}, nadj[NUM_TILE_CFG] = {};
 };
 
 namespace {
@@ -192,19 +192,14 @@ void fill_models(fkadj_instance *I)
     b.ghc = (float)a.ghc;
 }
 
-// Built-in tile table.  Large tiles cut the halo recomputation (x1.25 at 64x16, x1.15 at 64x32 for
-// the forward) but give fewer workgroups; a grid keeps the smaller tile unless the larger one still
-// yields at least two workgroups per CU.
-int default_fwd_cfg(int rows, int pitch)
-{
-    long tiles32 = (long)(pitch / FUSED_TX) * ((rows + 31) / 32);
-    return tiles32 >= 512 ? TILE_32_512 : TILE_16_256;
-}
-int default_adj_cfg(int rows, int pitch)
-{
-    long tiles32 = (long)(pitch / FUSED_TX) * ((rows + 31) / 32);
-    return tiles32 >= 512 ? TILE_32_512 : TILE_16_256;
-}
+// Built-in tile table, from measurements on MI355X at 512^2 with 1 and 16 instances:
+//  * adjoint: 64x16 cells per 512-thread workgroup (two tile cells per thread) beat 64x16/256,
+//    64x32/512 and 64x32/1024 at both sizes (occupancy matters more than the x1.5 halo work);
+//  * forward: 64x16/512 for a single instance, 64x32/512 once a launch covers about 2M cells
+//    (the smaller x1.15 halo recomputation pays when the GPU is full).  The forward tile does not
+//    change any result, so a group picks it from its total size.
+int default_adj_cfg(int, int) { return TILE_16_512; }
+int default_fwd_cfg(int64_t group_cells) { return group_cells >= (int64_t)2 << 20 ? TILE_32_512 : TILE_16_512; }
 
 // XCD-aware order: the GPU deals workgroups round-robin over its 8 XCDs, each with its own L2.
 // Block b runs on XCD b % 8; giving XCD x a contiguous run of tiles keeps neighbouring tiles
@@ -497,7 +492,7 @@ void group_free(FusedGroupDev *g)
 {
     if (!g) return;
     if (g->d_slots) (void)hipFree(g->d_slots);
-    for (int c = 0; c < 4; c++) {
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
         if (g->d_fwd[c]) (void)hipFree(g->d_fwd[c]);
         if (g->d_adj[c]) (void)hipFree(g->d_adj[c]);
     }
@@ -510,18 +505,21 @@ int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, Fus
     if (!g) return FKADJ_ERR_NOMEM;
     g->members = members;
     std::vector<FusedSlot> slots;
-    std::vector<TileTask> fl[4], al[4];
+    std::vector<TileTask> fl[NUM_TILE_CFG], al[NUM_TILE_CFG];
+    int64_t cells = 0;
+    for (auto *I : members) cells += (int64_t)I->rows * I->pitch;
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k];
         slots.push_back(I->slot_h);
-        auto ft = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(I->fwd_cfg));
-        fl[I->fwd_cfg].insert(fl[I->fwd_cfg].end(), ft.begin(), ft.end());
+        const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
+        auto ft = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(fc));
+        fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
         auto at = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(I->adj_cfg));
         al[I->adj_cfg].insert(al[I->adj_cfg].end(), at.begin(), at.end());
     }
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
-    for (int c = 0; c < 4 && ok; c++) {
+    for (int c = 0; c < NUM_TILE_CFG && ok; c++) {
         for (int side = 0; side < 2 && ok; side++) {
             std::vector<TileTask> v = xcd_order(side == 0 ? fl[c] : al[c]);
             if (v.empty()) continue;
@@ -539,13 +537,13 @@ int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, Fus
 
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
-    for (int c = 1; c < 4; c++)
+    for (int c = 1; c < NUM_TILE_CFG; c++)
         if (g->nfwd[c]) fused_forward_launch(c, g->d_slots, g->d_fwd[c], g->nfwd[c], mode, step, seg_a, s);
 }
 
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
-    for (int c = 1; c < 4; c++)
+    for (int c = 1; c < NUM_TILE_CFG; c++)
         if (g->nadj[c]) fused_adjoint_launch(c, g->d_slots, g->d_adj[c], g->nadj[c], i, seg_a, s);
 }
 
@@ -695,7 +693,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     *out = nullptr;
     if (p->nrows < 4 || p->ncols < 4 || p->nt < 1 || !(p->dt > 0) || !std::isfinite(p->dt)) return FKADJ_ERR_ARG;
     if (p->precision != FKADJ_F64 && p->precision != FKADJ_F32) return FKADJ_ERR_ARG;
-    if (p->fwd_tile < 0 || p->fwd_tile > 3 || (p->adj_tile != 0 && p->adj_tile != TILE_16_256 && p->adj_tile != TILE_32_512))
+    if (p->fwd_tile < 0 || p->fwd_tile >= NUM_TILE_CFG || p->adj_tile < 0 || p->adj_tile >= NUM_TILE_CFG)
         return FKADJ_ERR_ARG;
     if (p->pulse_mode == FKADJ_PULSE_KNOTS) {
         if (p->nt + 1 < 4) return FKADJ_ERR_ARG;
@@ -778,9 +776,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     const int nt = I->nt;
     I->nb_plain = (I->pitch / PLAIN_BX) * ((I->rows + PLAIN_BY - 1) / PLAIN_BY);
     if (I->prec == FKADJ_F32) {
-        I->fwd_cfg = (p->fwd_tile >= 1 && p->fwd_tile <= 3) ? p->fwd_tile : default_fwd_cfg(I->rows, I->pitch);
-        I->adj_cfg = (p->adj_tile == TILE_16_256 || p->adj_tile == TILE_32_512) ? p->adj_tile
-                                                                                 : default_adj_cfg(I->rows, I->pitch);
+        I->fwd_cfg = p->fwd_tile;   // 0: chosen per group from its total size
+        I->adj_cfg = p->adj_tile > 0 ? p->adj_tile : default_adj_cfg(I->rows, I->pitch);
         const int ty_a = tile_rows_of(I->adj_cfg);
         I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + ty_a - 1) / ty_a);   // adjoint tiles
     } else {

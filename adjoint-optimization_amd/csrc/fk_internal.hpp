@@ -56,9 +56,10 @@ void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
 // ---- fused FP32 path: one launch per time step over a tile work-list of one or more instances
 constexpr int FUSED_TX = 64;    // tile width: one wavefront row
 constexpr int FUSED_HALO = 4;   // one cell per RK stage
-// tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512
-constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3;
-__host__ __device__ constexpr int tile_rows_of(int cfg) { return cfg == TILE_16_256 ? 16 : 32; }
+// tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512, 4 = 16 x 512, 5 = 32 x 1024
+constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3, TILE_16_512 = 4, TILE_32_1024 = 5;
+constexpr int NUM_TILE_CFG = 6;
+__host__ __device__ constexpr int tile_rows_of(int cfg) { return (cfg == TILE_16_256 || cfg == TILE_16_512) ? 16 : 32; }
 
 struct FusedSlot {
     int rows, pitch, n;
@@ -91,7 +92,6 @@ enum FusedMode { MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2 };
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
                           int step, int seg_a, hipStream_t stream);
 //   adjoint step over [t_{i-1}, t_i] for global i (seg_a = first step of the current segment);
-//   cfg TILE_16_256 or TILE_32_512.
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
                           int seg_a, hipStream_t stream);
 

@@ -57,34 +57,68 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
 template <class G>
 __device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || (sl - G::TS + 1) <= 4 - k; }
 
-template <int RY, int RX>
-__device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], uint32_t c, int ly, int lx, float E,
-                               float gh, float *f, float *yc)
+// Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0 in
+// LDS, so the mirror Laplacian (a missing neighbour takes the opposite one's value) is
+// sum_w = (1+[b missing]) a + (1+[a missing]) b per axis, and L^T uses its 0/1/2 weights directly.
+struct LapW {
+    float t, b, l, r;   // weights of the top (i-1), bottom (i+1), left (j-1), right (j+1) values
+    float gh;           // 2 Du/h * ghost_lr
+};
+
+__device__ inline LapW lap_weights(const Model32 &m, uint32_t c)
 {
-    float l[3];
+    LapW w;
+    w.t = (c & C_BOT_MISS) ? 2.0f : 1.0f;
+    w.b = (c & C_TOP_MISS) ? 2.0f : 1.0f;
+    w.l = (c & C_RIGHT_MISS) ? 2.0f : 1.0f;
+    w.r = (c & C_LEFT_MISS) ? 2.0f : 1.0f;
+    w.gh = m.ghc * (float)code_gh(c);
+    return w;
+}
+
+__device__ inline LapW lapT_weights(uint32_t c)
+{
+    LapW w;
+    w.t = (float)((c >> C_WT_SHIFT) & 3u);
+    w.b = (float)((c >> C_WB_SHIFT) & 3u);
+    w.l = (float)((c >> C_WL_SHIFT) & 3u);
+    w.r = (float)((c >> C_WR_SHIFT) & 3u);
+    w.gh = 0.0f;
+    return w;
+}
+
+// sum of weighted neighbours minus 4 x centre (grid units)
+template <int RY, int RX>
+__device__ inline float stencil(const float (*s)[RY][RX], int v, int ly, int lx, const LapW &w, float ctr)
+{
+    return fmaf(w.t, s[v][ly - 1][lx], fmaf(w.b, s[v][ly + 1][lx],
+                fmaf(w.l, s[v][ly][lx - 1], fmaf(w.r, s[v][ly][lx + 1], -4.0f * ctr))));
+}
+
+// f(y) + E gh at an active cell from the LDS tile
+template <int RY, int RX>
+__device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], const LapW &w, int ly, int lx, float E,
+                               float *f, float *yc)
+{
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-        float ctr = s[v][ly][lx];
-        yc[v] = ctr;
-        l[v] = lap_val<float>(c, ctr, s[v][ly - 1][lx], s[v][ly + 1][lx], s[v][ly][lx - 1], s[v][ly][lx + 1], m.lc[v]);
-    }
+    for (int v = 0; v < 3; v++) yc[v] = s[v][ly][lx];
     Gates32 g = gates32(m, yc[0]);
     float fu, fv, fw;
     fion32(m, g, yc[0], yc[1], yc[2], fu, fv, fw);
-    f[0] = l[0] + fu + E * gh;
-    f[1] = l[1] + fv;
-    f[2] = l[2] + fw;
+    f[0] = fmaf(m.lc[0], stencil<RY, RX>(s, 0, ly, lx, w, yc[0]), fmaf(E, w.gh, fu));
+    f[1] = fmaf(m.lc[1], stencil<RY, RX>(s, 1, ly, lx, w, yc[1]), fv);
+    f[2] = fmaf(m.lc[2], stencil<RY, RX>(s, 2, ly, lx, w, yc[2]), fw);
 }
 
+// L^T z at an active cell from the LDS tile (missing neighbours have weight 0)
 template <int RY, int RX>
-__device__ inline void lapT_lds(const Model32 &m, const float (*s)[RY][RX], uint32_t c, int ly, int lx, float *lt,
+__device__ inline void lapT_lds(const Model32 &m, const float (*s)[RY][RX], const LapW &w, int ly, int lx, float *lt,
                                 float *zc)
 {
 #pragma unroll
     for (int v = 0; v < 3; v++) {
-        float ctr = s[v][ly][lx];
-        zc[v] = ctr;
-        lt[v] = lapT_val<float>(c, ctr, s[v][ly - 1][lx], s[v][ly + 1][lx], s[v][ly][lx - 1], s[v][ly][lx + 1], m.lc[v]);
+        zc[v] = s[v][ly][lx];
+        lt[v] = m.lc[v] * stencil<RY, RX>(s, v, ly, lx, w, zc[v]);
     }
 }
 
@@ -95,20 +129,29 @@ __device__ inline void jtz(const Jac32 &J, const float *z, float *o)
     o[2] = J.a20 * z[0] + J.a22 * z[2];
 }
 
+// Region (tile + 4-cell halo) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
+// first 64 cells of a row (coalesced), lanes 0..7 the last 8.  Cells outside the grid read as 0
+// with code 0 (inactive).
 template <class G>
 __device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
                                    float (*dst)[G::RY][G::RXP], uint16_t (*sc)[G::RXP])
 {
-    constexpr int RX = G::RX, RY = G::RY;
+    constexpr int RX = G::RX, RY = G::RY, NW = G::NT / 64;
+    const int lane = tid & 63, wave = tid >> 6;
     const int gx0 = x0 - FUSED_HALO, gy0 = y0 - FUSED_HALO;
-    for (int idx = tid; idx < RX * RY; idx += G::NT) {
-        int ly = idx / RX, lx = idx - ly * RX;
-        int gy = gy0 + ly, gx = gx0 + lx;
-        bool in = (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
-        int k = gy * S.pitch + gx;
+    static_assert(RX - 64 == 8, "tail loads assume an 8-cell halo");
+    auto cell = [&](int ly, int lx) {
+        const int gy = gy0 + ly, gx = gx0 + lx;
+        const bool in = (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
+        const int k = gy * S.pitch + gx;
         if (sc) sc[ly][lx] = in ? S.code[k] : (uint16_t)0;
 #pragma unroll
         for (int v = 0; v < 3; v++) dst[v][ly][lx] = in ? src[v * S.n + k] : 0.0f;
+    };
+    for (int ly = wave; ly < RY; ly += NW) cell(ly, lane);                    // first 64 cells of each row
+    for (int r0 = wave * 8; r0 < RY; r0 += NW * 8) {                           // last 8 cells, 8 rows per wave
+        const int ly = r0 + (lane >> 3);
+        if (ly < RY) cell(ly, 64 + (lane & 7));
     }
 }
 
@@ -160,12 +203,14 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
     int ly[NS], lx[NS];
     bool own[NS];
     uint32_t cc[NS];
+    LapW lw[NS];
     float y[NS][3], acc[G::TS][3], f[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
         own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
         cc[sl] = sC[ly[sl]][lx[sl]];
+        lw[sl] = lap_weights(m, cc[sl]);
 #pragma unroll
         for (int v = 0; v < 3; v++) y[sl][v] = sY[v][ly[sl]][lx[sl]];
     }
@@ -179,7 +224,7 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
             f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
             if (own[sl] && (cc[sl] & C_ACTIVE)) {
                 float yc[3];
-                rhs_lds<RY, RX>(m, sY, cc[sl], ly[sl], lx[sl], E, m.ghc * (float)code_gh(cc[sl]), f[sl], yc);
+                rhs_lds<RY, RX>(m, sY, lw[sl], ly[sl], lx[sl], E, f[sl], yc);
             }
         }
         if (k == 1 && kout) {
@@ -277,6 +322,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     int ly[NS], lx[NS], kk[NS];
     bool own[NS], act[NS];
     uint32_t cc[NS];
+    LapW tw[NS];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
         own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
@@ -284,6 +330,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         cc[sl] = sC[ly[sl]][lx[sl]];
         act[sl] = own[sl] && (cc[sl] & C_ACTIVE);
         kk[sl] = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
+        tw[sl] = lapT_weights(cc[sl]);
     }
 
     // phase 1a: y(t_{i-1}+dt/3) and the partial midpoint value on R1 from the stored end slopes
@@ -291,9 +338,6 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
         float y3[3] = {0.0f, 0.0f, 0.0f};
-#pragma unroll
-        for (int v = 0; v < 3; v++) { y4[sl][v] = 0.0f; yb[sl][v] = 0.0f; }
-        if (sl < TS) { ya[sl][0] = ya[sl][1] = ya[sl][2] = 0.0f; }
         if (act[sl]) {
             const int q = kk[sl];
             const float gh = m.ghc * (float)code_gh(cc[sl]);
@@ -323,7 +367,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         z[sl][0] = z[sl][1] = z[sl][2] = 0.0f;
         if (sl < TS) dz[sl][0] = dz[sl][1] = dz[sl][2] = 0.0f;
         if (!own[sl]) continue;
-        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, z[sl]);
+        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, z[sl]);
         if (act[sl]) {
             Gates32 g = gates32(m, yb[sl][0]);
             Jac32 J = jac32(m, g, yb[sl][0], yb[sl][1], yb[sl][2]);
@@ -352,10 +396,10 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
         if (!own[sl]) continue;
-        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             float f3[3], yc[3], yh[3];
-            rhs_lds<RY, RX>(m, sB, cc[sl], ly[sl], lx[sl], E3, m.ghc * (float)code_gh(cc[sl]), f3, yc);
+            rhs_lds<RY, RX>(m, sB, lap_weights(m, cc[sl]), ly[sl], lx[sl], E3, f3, yc);
 #pragma unroll
             for (int v = 0; v < 3; v++) yh[v] = y4[sl][v] + (float)(27.0 / 64.0) * (f3[v] * dt);
             Gates32 g = gates32(m, yh[0]);
@@ -384,7 +428,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
         if (!own[sl]) continue;
-        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             jtz(Jh[sl], zc, o);
 #pragma unroll
@@ -408,7 +452,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 #pragma unroll
     for (int sl = 0; sl < TS; sl++) {
         float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
-        lapT_lds<RY, RX>(m, sA, cc[sl], ly[sl], lx[sl], lt, zc);
+        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             Gates32 g = gates32(m, ya[sl][0]);
             Jac32 J = jac32(m, g, ya[sl][0], ya[sl][1], ya[sl][2]);
@@ -429,13 +473,16 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 }
 
 // ------------------------------------------------------------------------------ launchers
-// Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512.
+// Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
+// 4: 64x16 / 512, 5: 64x32 / 1024.
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
                           int step, int seg_a, hipStream_t stream)
 {
     switch (cfg) {
     case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
     case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
     default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
     }
 }
@@ -444,7 +491,10 @@ void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *t
                           int seg_a, hipStream_t stream)
 {
     switch (cfg) {
+    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
     case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
+    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
+    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
     default: hipLaunchKernelGGL((k_fused_adj<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
     }
 }

@@ -44,5 +44,15 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
+def build_copy_probe() -> str:
+    """The calibration copy kernel used for the HBM-traffic figures (measure/copy_probe.hip)."""
+    src = os.path.join(os.path.dirname(PKG), "measure", "copy_probe.hip")
+    out = src[:-4]
+    r = subprocess.run([HIPCC, *FLAGS, "-o", out, src], capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -96,9 +96,11 @@ def main():
     states = W.spiral_states(masks, seeds)
     log(f"[bench] states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
+    from fkadj import autotune
+    fwd_tile, adj_tile = autotune.lookup(N, B)
     insts = []
     for m, y in zip(masks, states):
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt)
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)
         inst.set_state(y)
         insts.append(inst)
 

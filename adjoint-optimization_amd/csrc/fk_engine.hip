@@ -14,6 +14,8 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <deque>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -113,6 +115,9 @@ struct fkadj_instance {
     Model32 m32{};
     std::vector<int32_t> cell_of;
     bool state_set = false;
+    bool ws_ready = false;   // checkpoint / segment / adjoint workspace allocated
+    bool lazy_ws = false;    // workspace is allocated on use and released after a served request
+    int busy = 0;            // requests of a server in flight on this instance
 
     uint16_t *d_code = nullptr;
     int32_t *d_cell_of = nullptr;
@@ -154,6 +159,11 @@ namespace {
 
 template <typename T>
 T *as(char *p) { return reinterpret_cast<T *>(p); }
+
+int ensure_workspace(fkadj_instance *I);
+void release_workspace(fkadj_instance *I);
+void group_free(FusedGroupDev *g);
+int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, FusedGroupDev **out);
 
 void fill_models(fkadj_instance *I)
 {
@@ -435,6 +445,55 @@ struct Sampler {
     }
 };
 
+// ------------------------------------------------------------------------ workspace
+// Checkpoints, segment states, forward ring, adjoint ping-pong and (FP32) stored slopes: the
+// memory a gradient evaluation needs beyond the resident problem.  Zeroed on allocation so hole
+// cells read 0 everywhere.
+int ensure_workspace(fkadj_instance *I)
+{
+    if (I->ws_ready) return FKADJ_OK;
+    const size_t st = I->st_bytes;
+    auto al = [&](char *&p, size_t bytes) -> bool {
+        if (hipMalloc((void **)&p, bytes) != hipSuccess) { p = nullptr; return false; }
+        return hipMemsetAsync(p, 0, bytes, I->stream) == hipSuccess;
+    };
+    bool ok = al(I->d_ck, st * I->nck) && al(I->d_seg, st * (I->K + 1)) && al(I->d_ring, st * 2) && al(I->d_z, st * 2);
+    if (ok && I->prec == FKADJ_F64) ok = al(I->d_work, st * 6);
+    if (ok && I->prec == FKADJ_F32) ok = al(I->d_kseg, st * 2 * I->K) && al(I->d_kT, st);
+    if (ok && I->prec == FKADJ_F32) {
+        FusedSlot &S = I->slot_h;
+        S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
+        S.nt = I->nt; S.K = I->K; S.nck = I->nck;
+        S.ntiles = I->nb_part;
+        S.code = I->d_code;
+        S.ck = as<float>(I->d_ck); S.seg = as<float>(I->d_seg); S.ring = as<float>(I->d_ring); S.zb = as<float>(I->d_z);
+        S.kseg = as<float>(I->d_kseg); S.kT = as<float>(I->d_kT);
+        S.Ef = I->d_Ef; S.Ea = I->d_Ea; S.partial = I->d_part;
+        S.m = I->m32; S.dt = (float)I->p.dt;
+        std::vector<fkadj_instance *> self{I};
+        ok = group_build(self, I->stream, &I->group) == FKADJ_OK;
+    }
+    ok = ok && hipStreamSynchronize(I->stream) == hipSuccess;
+    if (!ok) {
+        release_workspace(I);
+        return FKADJ_ERR_NOMEM;
+    }
+    I->ws_ready = true;
+    return FKADJ_OK;
+}
+
+void release_workspace(fkadj_instance *I)
+{
+    if (I->stream) (void)hipStreamSynchronize(I->stream);
+    for (char **p : {&I->d_ck, &I->d_seg, &I->d_ring, &I->d_z, &I->d_work, &I->d_kseg, &I->d_kT}) {
+        if (*p) (void)hipFree(*p);
+        *p = nullptr;
+    }
+    group_free(I->group);
+    I->group = nullptr;
+    I->ws_ready = false;
+}
+
 // ------------------------------------------------------------------------ one evaluation
 // Forward sweep with checkpoints, objective and terminal adjoint, then per segment (last to
 // first) recompute + adjoint steps.  All launches go to one stream.
@@ -501,6 +560,7 @@ void group_free(FusedGroupDev *g)
 
 int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, FusedGroupDev **out)
 {
+    if (members.empty() || members.size() > (size_t)MAX_SLOTS) return FKADJ_ERR_ARG;
     auto *g = new (std::nothrow) FusedGroupDev();
     if (!g) return FKADJ_ERR_NOMEM;
     g->members = members;
@@ -535,16 +595,25 @@ int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, Fus
     return FKADJ_OK;
 }
 
+Cursors uniform_cursors(int count, int mode, int step, int seg_a)
+{
+    Cursors c;
+    for (int k = 0; k < MAX_SLOTS; k++) c.c[k] = SlotCursor{k < count ? mode : MODE_IDLE, step, seg_a, 0};
+    return c;
+}
+
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
-    for (int c = 1; c < NUM_TILE_CFG; c++)
-        if (g->nfwd[c]) fused_forward_launch(c, g->d_slots, g->d_fwd[c], g->nfwd[c], mode, step, seg_a, s);
+    const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
+    for (int k = 1; k < NUM_TILE_CFG; k++)
+        if (g->nfwd[k]) fused_forward_launch(k, g->d_slots, g->d_fwd[k], g->nfwd[k], c, s);
 }
 
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
-    for (int c = 1; c < NUM_TILE_CFG; c++)
-        if (g->nadj[c]) fused_adjoint_launch(c, g->d_slots, g->d_adj[c], g->nadj[c], i, seg_a, s);
+    const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
+    for (int k = 1; k < NUM_TILE_CFG; k++)
+        if (g->nadj[k]) fused_adjoint_launch(k, g->d_slots, g->d_adj[k], g->nadj[k], c, s);
 }
 
 int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
@@ -600,7 +669,9 @@ int fetch_and_assemble(fkadj_instance *I, hipStream_t s, const double *E, double
 int eval_one(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
 {
     if (!I->state_set) return FKADJ_ERR_STATE;
+    if (I->busy) return FKADJ_ERR_BUSY;
     if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    CHK(ensure_workspace(I));
     CHK(build_tables(I, E));
     Sampler smp(g_profiling.load() != 0);
     hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
@@ -787,15 +858,6 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     DMALLOC(I->d_cell_of, sizeof(int32_t) * I->n_num);
     DMALLOC(I->d_gamma, sizeof(double) * 3);
     DMALLOC(I->d_y0, st);
-    DMALLOC(I->d_ck, st * I->nck);
-    DMALLOC(I->d_seg, st * (I->K + 1));
-    DMALLOC(I->d_ring, st * 2);
-    DMALLOC(I->d_z, st * 2);
-    if (I->prec == FKADJ_F64) DMALLOC(I->d_work, st * 6);
-    if (I->prec == FKADJ_F32) {
-        DMALLOC(I->d_kseg, st * 2 * I->K);
-        DMALLOC(I->d_kT, st);
-    }
     DMALLOC(I->d_Ef, sizeof(double) * 3 * nt);
     DMALLOC(I->d_Ea, sizeof(double) * 3 * nt);
     DMALLOC(I->d_part, sizeof(double) * (size_t)nt * I->nb_part);
@@ -810,12 +872,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
         DMALLOC(I->d_colind, sizeof(int32_t) * std::max<size_t>(1, colind.size()));
         DMALLOC(I->d_val, sizeof(double) * std::max<size_t>(1, val.size()));
     }
-    // zero every state buffer once: hole cells stay zero
-    if (hipMemsetAsync(I->d_y0, 0, st, I->stream) != hipSuccess || hipMemsetAsync(I->d_ck, 0, st * I->nck, I->stream) != hipSuccess ||
-        hipMemsetAsync(I->d_seg, 0, st * (I->K + 1), I->stream) != hipSuccess ||
-        hipMemsetAsync(I->d_ring, 0, st * 2, I->stream) != hipSuccess || hipMemsetAsync(I->d_z, 0, st * 2, I->stream) != hipSuccess)
-        return fail(FKADJ_ERR_DEVICE);
-    if (I->d_work && hipMemsetAsync(I->d_work, 0, st * 6, I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
+    // hole cells of every state buffer stay zero
+    if (hipMemsetAsync(I->d_y0, 0, st, I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
     bool ok = hipMemcpyAsync(I->d_code, code.data(), sizeof(uint16_t) * I->n, hipMemcpyHostToDevice, I->stream) == hipSuccess &&
               hipMemcpyAsync(I->d_cell_of, I->cell_of.data(), sizeof(int32_t) * I->n_num, hipMemcpyHostToDevice, I->stream) == hipSuccess &&
               hipMemcpyAsync(I->d_gamma, p->gamma, sizeof(double) * 3, hipMemcpyHostToDevice, I->stream) == hipSuccess;
@@ -824,19 +882,10 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
              (colind.empty() || (hipMemcpyAsync(I->d_colind, colind.data(), sizeof(int32_t) * colind.size(), hipMemcpyHostToDevice, I->stream) == hipSuccess &&
                                  hipMemcpyAsync(I->d_val, val.data(), sizeof(double) * val.size(), hipMemcpyHostToDevice, I->stream) == hipSuccess));
     if (!ok) return fail(FKADJ_ERR_DEVICE);
-    if (I->prec == FKADJ_F32) {
-        FusedSlot &S = I->slot_h;
-        S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
-        S.nt = nt; S.K = I->K; S.nck = I->nck;
-        S.ntiles = I->nb_part;
-        S.code = I->d_code;
-        S.ck = as<float>(I->d_ck); S.seg = as<float>(I->d_seg); S.ring = as<float>(I->d_ring); S.zb = as<float>(I->d_z);
-        S.kseg = as<float>(I->d_kseg); S.kT = as<float>(I->d_kT);
-        S.Ef = I->d_Ef; S.Ea = I->d_Ea; S.partial = I->d_part;
-        S.m = I->m32; S.dt = (float)p->dt;
-        std::vector<fkadj_instance *> self{I};
-        int grc = group_build(self, I->stream, &I->group);
-        if (grc) return fail(grc);
+    I->lazy_ws = p->lazy_workspace != 0;
+    if (!I->lazy_ws) {
+        int wrc = ensure_workspace(I);
+        if (wrc) return fail(wrc);
     }
 #undef DMALLOC
     if (hipStreamSynchronize(I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
@@ -847,11 +896,13 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
 int fkadj_destroy(fkadj_instance *I)
 {
     if (!I) return FKADJ_ERR_ARG;
+    if (I->busy) return FKADJ_ERR_BUSY;
     if (I->stream) (void)hipStreamSynchronize(I->stream);
     dfree(I->d_code); dfree(I->d_cell_of); dfree(I->d_rowptr); dfree(I->d_colind); dfree(I->d_val); dfree(I->d_gamma);
-    dfree(I->d_kseg); dfree(I->d_kT); dfree(I->d_y0); dfree(I->d_ck); dfree(I->d_seg); dfree(I->d_ring); dfree(I->d_z); dfree(I->d_work);
+    release_workspace(I);
+    dfree(I->d_y0);
     dfree(I->d_Ef); dfree(I->d_Ea); dfree(I->d_part); dfree(I->d_partT); dfree(I->d_zgh); dfree(I->d_L0part);
-    dfree(I->d_L0); dfree(I->d_packed); dfree(I->d_mask); group_free(I->group); I->group = nullptr;
+    dfree(I->d_L0); dfree(I->d_packed); dfree(I->d_mask);
     if (I->stream) (void)hipStreamDestroy(I->stream);
     delete I;
     return FKADJ_OK;
@@ -906,7 +957,9 @@ int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every
     if (!I || !E || nsteps < 0 || nsteps > I->nt) return FKADJ_ERR_ARG;
     if (snap_every > 0 && !snaps && nsteps / snap_every > 1) return FKADJ_ERR_ARG;
     if (!I->state_set) return FKADJ_ERR_STATE;
+    if (I->busy) return FKADJ_ERR_BUSY;
     if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    CHK(ensure_workspace(I));
     CHK(build_tables(I, E));
     hipStream_t s = I->stream;
     HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
@@ -934,7 +987,7 @@ int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every
 
 int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *const *E, int nsteps)
 {
-    if (count < 1 || !insts || !E || nsteps < 0) return FKADJ_ERR_ARG;
+    if (count < 1 || count > MAX_SLOTS || !insts || !E || nsteps < 0) return FKADJ_ERR_ARG;
     for (int k = 0; k < count; k++) {
         fkadj_instance *I = insts[k];
         if (!I || !E[k] || I->prec != FKADJ_F32 || I->nt != insts[0]->nt || I->K != insts[0]->K || nsteps > I->nt)
@@ -942,8 +995,10 @@ int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *c
         for (int q = 0; q < k; q++)
             if (insts[q] == I) return FKADJ_ERR_ARG;
         if (!I->state_set) return FKADJ_ERR_STATE;
+        if (I->busy) return FKADJ_ERR_BUSY;
         if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
     }
+    for (int k = 0; k < count; k++) CHK(ensure_workspace(insts[k]));
     hipStream_t s = insts[0]->stream;
     for (int k = 0; k < count; k++) {
         CHK(build_tables(insts[k], E[k]));
@@ -1037,7 +1092,7 @@ int fkadj_last_timing(fkadj_timing *t)
 int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
                      double *Lcost, double *L0, double *LE)
 {
-    if (count < 1 || !insts || !E) return FKADJ_ERR_ARG;
+    if (count < 1 || count > MAX_SLOTS || !insts || !E) return FKADJ_ERR_ARG;
     for (int k = 0; k < count; k++) {
         fkadj_instance *I = insts[k];
         if (!I || !E[k]) return FKADJ_ERR_ARG;
@@ -1045,8 +1100,10 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
         for (int q = 0; q < k; q++)
             if (insts[q] == I) return FKADJ_ERR_ARG;
         if (!I->state_set) return FKADJ_ERR_STATE;
+        if (I->busy) return FKADJ_ERR_BUSY;
         if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
     }
+    for (int k = 0; k < count; k++) CHK(ensure_workspace(insts[k]));
     hipStream_t s = insts[0]->stream;
     for (int k = 0; k < count; k++) {
         CHK(build_tables(insts[k], E[k]));
@@ -1073,3 +1130,408 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
 
 }  // extern "C"
 
+// ============================================================================ serving
+// Continuous batching on one stream.  Every admitted request owns a slot; each round issues one
+// forward-kind launch (slots in their forward or recompute phase) and one adjoint launch (slots in
+// an adjoint phase), each slot at its own step, through per-slot cursors.  A request's phase
+// machine: forward steps 0..nt-1 -> objective, terminal z.gh, f(y_nt) -> for each segment from the
+// last: recompute (b-a steps) then adjoint (i = b..a+1) -> fixed-order sums and an async copy of
+// z.gh and L0 to pinned host memory.  Slots free and new requests join at round boundaries.  The
+// per-request arithmetic is that of fkadj_grad on the same instance, so every result equals the
+// request evaluated alone.
+namespace {
+
+enum { RQ_QUEUED, RQ_RUNNING, RQ_FINISHING, RQ_DONE, RQ_CANCELLED, RQ_FAILED };
+enum { PH_FWD, PH_REC, PH_ADJ };
+constexpr int kRoundsInFlight = 8;
+
+struct ServReq {
+    int64_t id = 0;
+    fkadj_instance *I = nullptr;
+    std::vector<double> E;
+    int state = RQ_QUEUED;
+    int rc = FKADJ_OK;
+    int slot = -1;
+    int phase = PH_FWD, step = 0, seg = 0;
+    bool cancel = false;
+    int64_t bytes = 0;
+    double *h_out = nullptr;     // pinned: zgh[nt+1], L0
+    hipEvent_t done = nullptr;
+    std::vector<double> grad;
+    double Lc = 0, L0 = 0, LE = 0;
+};
+
+}  // namespace
+
+struct fkadj_server {
+    int64_t budget = 0, used = 0;
+    int max_slots = 0;
+    hipStream_t stream = nullptr;
+    FusedSlot *d_slots = nullptr;
+    std::vector<ServReq *> slot_req;
+    TileTask *d_fwd[NUM_TILE_CFG] = {}, *d_adj[NUM_TILE_CFG] = {};
+    int nfwd[NUM_TILE_CFG] = {}, nadj[NUM_TILE_CFG] = {};
+    size_t cap_fwd[NUM_TILE_CFG] = {}, cap_adj[NUM_TILE_CFG] = {};
+    bool dirty = false;
+    std::deque<ServReq *> queue;
+    std::vector<ServReq *> finishing;
+    std::map<int64_t, ServReq *> reqs;
+    int64_t next_id = 1;
+    hipEvent_t round_ev[kRoundsInFlight] = {};
+    int64_t rounds = 0;
+};
+
+namespace {
+
+bool instance_busy(const fkadj_instance *I) { return I->busy > 0; }
+
+// Task lists and slot table for the occupied slots (after the stream has drained).
+int server_rebuild(fkadj_server *S)
+{
+    HIPCHK(hipStreamSynchronize(S->stream));
+    std::vector<FusedSlot> slots(S->max_slots);
+    std::vector<TileTask> fl[NUM_TILE_CFG], al[NUM_TILE_CFG];
+    int64_t cells = 0;
+    for (int k = 0; k < S->max_slots; k++)
+        if (S->slot_req[k]) cells += (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
+    for (int k = 0; k < S->max_slots; k++) {
+        ServReq *r = S->slot_req[k];
+        if (!r) continue;
+        fkadj_instance *I = r->I;
+        slots[k] = I->slot_h;
+        const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
+        auto ft = make_tiles(k, I->rows, I->pitch, tile_rows_of(fc));
+        fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
+        auto at = make_tiles(k, I->rows, I->pitch, tile_rows_of(I->adj_cfg));
+        al[I->adj_cfg].insert(al[I->adj_cfg].end(), at.begin(), at.end());
+    }
+    HIPCHK(hipMemcpy(S->d_slots, slots.data(), sizeof(FusedSlot) * S->max_slots, hipMemcpyHostToDevice));
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
+        for (int side = 0; side < 2; side++) {
+            std::vector<TileTask> v = xcd_order(side == 0 ? fl[c] : al[c]);
+            TileTask *&d = side == 0 ? S->d_fwd[c] : S->d_adj[c];
+            size_t &cap = side == 0 ? S->cap_fwd[c] : S->cap_adj[c];
+            (side == 0 ? S->nfwd[c] : S->nadj[c]) = (int)v.size();
+            if (v.empty()) continue;
+            if (v.size() > cap) {
+                if (d) (void)hipFree(d);
+                d = nullptr;
+                cap = 0;
+                if (hipMalloc((void **)&d, sizeof(TileTask) * v.size()) != hipSuccess) return FKADJ_ERR_NOMEM;
+                cap = v.size();
+            }
+            HIPCHK(hipMemcpy(d, v.data(), sizeof(TileTask) * v.size(), hipMemcpyHostToDevice));
+        }
+    }
+    S->dirty = false;
+    return FKADJ_OK;
+}
+
+// Retire a running request from its slot: results (or nothing, if cancelled) are copied out
+// asynchronously; the slot is free for the next round.
+int server_finish(fkadj_server *S, ServReq *r, bool cancelled)
+{
+    fkadj_instance *I = r->I;
+    if (!cancelled) {
+        sum_rows_launch(I->d_part, I->nb_part, I->nt, I->d_zgh, S->stream);
+        HIPCHK(hipMemcpyAsync(r->h_out, I->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, S->stream));
+        HIPCHK(hipMemcpyAsync(r->h_out + I->nt + 1, I->d_L0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+    } else {
+        r->rc = FKADJ_ERR_CANCELLED;
+    }
+    HIPCHK(hipEventRecord(r->done, S->stream));
+    S->slot_req[r->slot] = nullptr;
+    r->slot = -1;
+    r->state = RQ_FINISHING;
+    S->finishing.push_back(r);
+    S->dirty = true;
+    return FKADJ_OK;
+}
+
+// Completed requests: assemble results, return budget and (lazy) workspace.
+int server_reap(fkadj_server *S, bool wait)
+{
+    int n = 0;
+    for (size_t k = 0; k < S->finishing.size();) {
+        ServReq *r = S->finishing[k];
+        hipError_t q = wait ? hipEventSynchronize(r->done) : hipEventQuery(r->done);
+        if (q == hipErrorNotReady) { k++; continue; }
+        if (q != hipSuccess && r->rc == FKADJ_OK) r->rc = FKADJ_ERR_DEVICE;
+        fkadj_instance *I = r->I;
+        if (r->rc == FKADJ_OK) {
+            const double *zgh = r->h_out;
+            const double L0v = r->h_out[I->nt + 1];
+            if (!all_finite(zgh, I->nt + 1) || !std::isfinite(L0v)) {
+                r->rc = FKADJ_ERR_NONFINITE;
+            } else {
+                r->grad.resize(I->ncontrols);
+                assemble(I, r->E.data(), zgh, L0v, r->grad.data(), &r->Lc, &r->L0, &r->LE);
+            }
+        }
+        r->state = r->rc == FKADJ_OK ? RQ_DONE : (r->rc == FKADJ_ERR_CANCELLED ? RQ_CANCELLED : RQ_FAILED);
+        if (r->h_out) (void)hipHostFree(r->h_out);
+        r->h_out = nullptr;
+        S->used -= r->bytes;
+        I->busy--;
+        if (I->lazy_ws && !I->busy) release_workspace(I);
+        S->finishing.erase(S->finishing.begin() + k);
+        n++;
+    }
+    return n;
+}
+
+// Admit queued requests while slots and budget allow (FIFO; a request whose instance is busy
+// lets later requests of other instances go first).
+int server_admit(fkadj_server *S)
+{
+    for (size_t q = 0; q < S->queue.size();) {
+        int free_slot = -1;
+        for (int k = 0; k < S->max_slots; k++)
+            if (!S->slot_req[k]) { free_slot = k; break; }
+        if (free_slot < 0) return FKADJ_OK;
+        ServReq *r = S->queue[q];
+        fkadj_instance *I = r->I;
+        if (instance_busy(I)) { q++; continue; }
+        const int64_t bytes = fkadj_footprint_bytes(&I->p);
+        if (bytes > S->budget) {   // can never fit: fail it instead of blocking the queue
+            r->rc = FKADJ_ERR_NOMEM;
+            r->state = RQ_FAILED;
+            S->queue.erase(S->queue.begin() + q);
+            continue;
+        }
+        if (S->used + bytes > S->budget) return FKADJ_OK;
+        int rc = ensure_workspace(I);
+        if (rc == FKADJ_OK) rc = build_tables(I, r->E.data());
+        if (rc == FKADJ_OK && hipStreamSynchronize(I->stream) != hipSuccess) rc = FKADJ_ERR_DEVICE;
+        if (rc == FKADJ_OK && hipHostMalloc((void **)&r->h_out, sizeof(double) * (I->nt + 2)) != hipSuccess)
+            rc = FKADJ_ERR_NOMEM;
+        if (rc == FKADJ_OK && hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess)
+            rc = FKADJ_ERR_DEVICE;
+        if (rc == FKADJ_OK &&
+            hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, S->stream) != hipSuccess)
+            rc = FKADJ_ERR_DEVICE;
+        S->queue.erase(S->queue.begin() + q);
+        if (rc != FKADJ_OK) {
+            r->rc = rc;
+            r->state = RQ_FAILED;
+            if (r->h_out) (void)hipHostFree(r->h_out);
+            r->h_out = nullptr;
+            if (I->lazy_ws && !I->busy) release_workspace(I);
+            continue;
+        }
+        I->busy++;
+        S->used += bytes;
+        r->bytes = bytes;
+        r->slot = free_slot;
+        r->state = RQ_RUNNING;
+        r->phase = PH_FWD;
+        r->step = 0;
+        S->slot_req[free_slot] = r;
+        S->dirty = true;
+    }
+    return FKADJ_OK;
+}
+
+// One round: one forward-kind and one adjoint launch over all busy slots, then every slot's phase
+// machine advances by one step.
+int server_round(fkadj_server *S)
+{
+    Cursors fc, ac, rc_rhs;
+    for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
+    bool any_f = false, any_a = false;
+    for (int k = 0; k < S->max_slots; k++) {
+        ServReq *r = S->slot_req[k];
+        if (!r) continue;
+        const int K = r->I->K;
+        if (r->phase == PH_FWD) { fc.c[k] = SlotCursor{MODE_FWD, r->step, 0, 0}; any_f = true; }
+        else if (r->phase == PH_REC) { fc.c[k] = SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0}; any_f = true; }
+        else { ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0}; any_a = true; }
+    }
+    if (any_f)
+        for (int c = 1; c < NUM_TILE_CFG; c++)
+            if (S->nfwd[c]) fused_forward_launch(c, S->d_slots, S->d_fwd[c], S->nfwd[c], fc, S->stream);
+    if (any_a)
+        for (int c = 1; c < NUM_TILE_CFG; c++)
+            if (S->nadj[c]) fused_adjoint_launch(c, S->d_slots, S->d_adj[c], S->nadj[c], ac, S->stream);
+    HIPCHK(hipGetLastError());
+    bool any_rhs = false;
+    for (int k = 0; k < S->max_slots; k++) {
+        ServReq *r = S->slot_req[k];
+        if (!r) continue;
+        fkadj_instance *I = r->I;
+        const int nt = I->nt, K = I->K;
+        if (r->phase == PH_FWD) {
+            if (++r->step == nt) {
+                terminal<float>(I, S->stream);                  // objective, z(T), z.gh at T
+                rc_rhs.c[k] = SlotCursor{MODE_RHS, nt - 1, 0, 0};
+                any_rhs = true;
+                r->phase = PH_REC;
+                r->seg = (nt + K - 1) / K - 1;
+                r->step = 0;
+            }
+        } else if (r->phase == PH_REC) {
+            const int a = r->seg * K, b = std::min(a + K, nt);
+            if (++r->step == b - a) { r->phase = PH_ADJ; r->step = b; }
+        } else {
+            const int a = r->seg * K;
+            if (--r->step == a) {
+                if (--r->seg < 0) CHK(server_finish(S, r, false));
+                else { r->phase = PH_REC; r->step = 0; }
+            }
+        }
+    }
+    if (any_rhs)
+        for (int c = 1; c < NUM_TILE_CFG; c++)
+            if (S->nfwd[c]) fused_forward_launch(c, S->d_slots, S->d_fwd[c], S->nfwd[c], rc_rhs, S->stream);
+    HIPCHK(hipGetLastError());
+    // cancellations take effect at the round boundary
+    for (int k = 0; k < S->max_slots; k++) {
+        ServReq *r = S->slot_req[k];
+        if (r && r->cancel) CHK(server_finish(S, r, true));
+    }
+    return FKADJ_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fkadj_server_create(int64_t budget_bytes, int max_slots, fkadj_server **out)
+{
+    if (!out || budget_bytes <= 0 || max_slots < 1 || max_slots > MAX_SLOTS) return FKADJ_ERR_ARG;
+    auto *S = new (std::nothrow) fkadj_server();
+    if (!S) return FKADJ_ERR_NOMEM;
+    S->budget = budget_bytes;
+    S->max_slots = max_slots;
+    S->slot_req.assign(max_slots, nullptr);
+    bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
+              hipMalloc((void **)&S->d_slots, sizeof(FusedSlot) * max_slots) == hipSuccess;
+    for (int k = 0; k < kRoundsInFlight && ok; k++)
+        ok = hipEventCreateWithFlags(&S->round_ev[k], hipEventDisableTiming) == hipSuccess;
+    if (!ok) {
+        fkadj_server_destroy(S);
+        return FKADJ_ERR_DEVICE;
+    }
+    *out = S;
+    return FKADJ_OK;
+}
+
+int fkadj_server_destroy(fkadj_server *S)
+{
+    if (!S) return FKADJ_ERR_ARG;
+    if (S->stream) (void)hipStreamSynchronize(S->stream);
+    for (int k = 0; k < S->max_slots; k++)
+        if (S->slot_req[k]) (void)server_finish(S, S->slot_req[k], true);
+    (void)server_reap(S, true);
+    for (auto &kv : S->reqs) {
+        ServReq *r = kv.second;
+        if (r->h_out) (void)hipHostFree(r->h_out);
+        if (r->done) (void)hipEventDestroy(r->done);
+        delete r;
+    }
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
+        if (S->d_fwd[c]) (void)hipFree(S->d_fwd[c]);
+        if (S->d_adj[c]) (void)hipFree(S->d_adj[c]);
+    }
+    if (S->d_slots) (void)hipFree(S->d_slots);
+    for (int k = 0; k < kRoundsInFlight; k++)
+        if (S->round_ev[k]) (void)hipEventDestroy(S->round_ev[k]);
+    if (S->stream) (void)hipStreamDestroy(S->stream);
+    delete S;
+    return FKADJ_OK;
+}
+
+int fkadj_server_submit(fkadj_server *S, fkadj_instance *I, const double *E, int64_t *request_id)
+{
+    if (!S || !I || !E || !request_id) return FKADJ_ERR_ARG;
+    if (I->prec != FKADJ_F32) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    auto *r = new (std::nothrow) ServReq();
+    if (!r) return FKADJ_ERR_NOMEM;
+    r->id = S->next_id++;
+    r->I = I;
+    r->E.assign(E, E + I->ncontrols);
+    S->reqs[r->id] = r;
+    S->queue.push_back(r);
+    *request_id = r->id;
+    return FKADJ_OK;
+}
+
+int fkadj_server_cancel(fkadj_server *S, int64_t id)
+{
+    if (!S) return FKADJ_ERR_ARG;
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return FKADJ_ERR_NOTFOUND;
+    ServReq *r = it->second;
+    if (r->state == RQ_QUEUED) {
+        for (size_t q = 0; q < S->queue.size(); q++)
+            if (S->queue[q] == r) { S->queue.erase(S->queue.begin() + q); break; }
+        r->rc = FKADJ_ERR_CANCELLED;
+        r->state = RQ_CANCELLED;
+    } else if (r->state == RQ_RUNNING) {
+        r->cancel = true;
+    }
+    return FKADJ_OK;
+}
+
+int fkadj_server_run(fkadj_server *S, int max_rounds)
+{
+    if (!S || max_rounds < 0) return FKADJ_ERR_ARG;
+    int completed = 0;
+    for (int round = 0; max_rounds == 0 || round < max_rounds; round++) {
+        completed += server_reap(S, false);
+        CHK(server_admit(S));
+        bool running = false;
+        for (auto *r : S->slot_req) running |= (r != nullptr);
+        if (!running) {
+            if (!S->finishing.empty()) {
+                completed += server_reap(S, true);
+                continue;
+            }
+            break;
+        }
+        if (S->dirty) CHK(server_rebuild(S));
+        if (S->rounds >= kRoundsInFlight) HIPCHK(hipEventSynchronize(S->round_ev[S->rounds % kRoundsInFlight]));
+        CHK(server_round(S));
+        HIPCHK(hipEventRecord(S->round_ev[S->rounds % kRoundsInFlight], S->stream));
+        S->rounds++;
+        if (max_rounds == 0 && completed > 0) break;
+    }
+    completed += server_reap(S, false);
+    return completed;
+}
+
+int fkadj_server_poll(fkadj_server *S, int64_t id, double *grad, double *Lcost, double *L0, double *LE)
+{
+    if (!S) return FKADJ_ERR_ARG;
+    (void)server_reap(S, false);
+    auto it = S->reqs.find(id);
+    if (it == S->reqs.end()) return FKADJ_ERR_NOTFOUND;
+    ServReq *r = it->second;
+    if (r->state == RQ_QUEUED || r->state == RQ_RUNNING || r->state == RQ_FINISHING) return FKADJ_ERR_BUSY;
+    int rc = r->rc;
+    if (rc == FKADJ_OK) {
+        if (grad) std::copy(r->grad.begin(), r->grad.end(), grad);
+        if (Lcost) *Lcost = r->Lc;
+        if (L0) *L0 = r->L0;
+        if (LE) *LE = r->LE;
+    }
+    if (r->done) (void)hipEventDestroy(r->done);
+    delete r;
+    S->reqs.erase(it);
+    return rc;
+}
+
+int fkadj_server_pending(const fkadj_server *S)
+{
+    if (!S) return FKADJ_ERR_ARG;
+    int n = 0;
+    for (auto &kv : S->reqs) {
+        int st = kv.second->state;
+        n += (st == RQ_QUEUED || st == RQ_RUNNING || st == RQ_FINISHING);
+    }
+    return n;
+}
+
+}  // extern "C"

@@ -83,17 +83,27 @@ struct TileTask {
 };
 
 // MODE_FWD: forward sweep (checkpoints + ring); MODE_REC: segment recompute, also stores the
-// stage-1 slopes for the adjoint's Hermite reconstruction; MODE_RHS: only f(y_nt) -> kT.
-enum FusedMode { MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2 };
+// stage-1 slopes for the adjoint's Hermite reconstruction; MODE_RHS: only f(y_nt) -> kT;
+// MODE_ADJ: adjoint step; MODE_IDLE: the slot's tiles do nothing in this launch.
+enum FusedMode { MODE_IDLE = -1, MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2, MODE_ADJ = 3 };
 
-// Launchers (stream-ordered, no synchronisation); cfg is a tile configuration.
-//   forward step `step` (0-based) of a sweep: MODE_FWD reads/writes ring/checkpoints,
-//   MODE_REC recomputes inside segment `seg_a` (first step of the segment) into seg[].
-void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
-                          int step, int seg_a, hipStream_t stream);
-//   adjoint step over [t_{i-1}, t_i] for global i (seg_a = first step of the current segment);
-void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
-                          int seg_a, hipStream_t stream);
+// Per-slot work of one launch, passed by value (1 KiB of kernel arguments): slots of one launch
+// may be at different steps and phases (continuous batching).  step is the forward step index
+// (MODE_FWD/REC) or the adjoint interval end i (MODE_ADJ); seg_a is the first step of the segment.
+constexpr int MAX_SLOTS = 64;
+struct SlotCursor {
+    int mode, step, seg_a, pad;
+};
+struct Cursors {
+    SlotCursor c[MAX_SLOTS];
+};
+
+// Launchers (stream-ordered, no synchronisation); cfg is a tile configuration.  Tiles whose
+// slot cursor is MODE_IDLE (or not the kernel's kind) return at once.
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          const Cursors &cur, hipStream_t stream);
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          const Cursors &cur, hipStream_t stream);
 
 // ---- common small kernels (fk_kernels_common.hip)
 // z (terminal adjoint, T planes) and L0 partials from y(T): CSR S over packed indices or s_diag*I.

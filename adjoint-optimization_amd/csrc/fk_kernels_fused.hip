@@ -158,8 +158,7 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 // ------------------------------------------------------------------------------ forward
 template <int TY, int NT>
 __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ slots,
-                                                        const TileTask *__restrict__ tasks, int mode, int step,
-                                                        int seg_a)
+                                                  const TileTask *__restrict__ tasks, Cursors cur)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT;
@@ -167,6 +166,9 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
     __shared__ uint16_t sC[RY][RX];
 
     const TileTask t = tasks[blockIdx.x];
+    const SlotCursor sc = cur.c[t.slot];
+    if (sc.mode != MODE_FWD && sc.mode != MODE_REC && sc.mode != MODE_RHS) return;   // uniform per block
+    const int mode = sc.mode, step = sc.step, seg_a = sc.seg_a;
     const FusedSlot &S = slots[t.slot];
     const int tid = threadIdx.x;
     const size_t st = (size_t)3 * S.n;
@@ -285,7 +287,7 @@ __device__ inline T h4p(T x0, T x1, T m0, T m1)
 
 template <int TY, int NT>
 __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ slots,
-                                                  const TileTask *__restrict__ tasks, int i, int seg_a)
+                                                  const TileTask *__restrict__ tasks, Cursors cur)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
@@ -294,6 +296,9 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     __shared__ uint16_t sC[RY][RX];
 
     const TileTask t = tasks[blockIdx.x];
+    const SlotCursor sc = cur.c[t.slot];
+    if (sc.mode != MODE_ADJ) return;   // uniform per block
+    const int i = sc.step, seg_a = sc.seg_a;
     const FusedSlot &S = slots[t.slot];
     const int tid = threadIdx.x;
     const size_t st = (size_t)3 * S.n;
@@ -475,27 +480,27 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 // ------------------------------------------------------------------------------ launchers
 // Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
 // 4: 64x16 / 512, 5: 64x32 / 1024.
-void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int mode,
-                          int step, int seg_a, hipStream_t stream)
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          const Cursors &cur, hipStream_t stream)
 {
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
-    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
-    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
-    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
-    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, mode, step, seg_a); break;
+    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, cur); break;
+    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
     }
 }
 
-void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks, int i,
-                          int seg_a, hipStream_t stream)
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
+                          const Cursors &cur, hipStream_t stream)
 {
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
-    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
-    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
-    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
-    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, i, seg_a); break;
+    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, cur); break;
+    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
     }
 }
 

@@ -39,6 +39,7 @@ class Problem(C.Structure):
         ("S_colptr", C.POINTER(C.c_int64)), ("S_rowind", C.POINTER(C.c_int64)), ("S_val", C.POINTER(C.c_double)),
         ("s_diag", C.c_double),
         ("precision", C.c_int), ("ckpt", C.c_int), ("fwd_tile", C.c_int), ("adj_tile", C.c_int),
+        ("lazy_workspace", C.c_int),
     ]
 
 
@@ -86,6 +87,13 @@ def lib():
     L.fkadj_stimulate.argtypes = [vp, C.POINTER(C.c_uint8), C.c_double]
     L.fkadj_batch_grad.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
     L.fkadj_batch_forward.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.c_int]
+    L.fkadj_server_create.argtypes = [C.c_int64, C.c_int, C.POINTER(vp)]
+    L.fkadj_server_destroy.argtypes = [vp]
+    L.fkadj_server_submit.argtypes = [vp, vp, dp, C.POINTER(C.c_int64)]
+    L.fkadj_server_cancel.argtypes = [vp, C.c_int64]
+    L.fkadj_server_run.argtypes = [vp, C.c_int]
+    L.fkadj_server_poll.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
+    L.fkadj_server_pending.argtypes = [vp]
     L.fkadj_set_profiling.argtypes = [C.c_int]
     L.fkadj_last_timing.argtypes = [C.POINTER(Timing)]
     _lib = L

@@ -58,7 +58,7 @@ class Instance:
 
     def __init__(self, active_index, nrows, ncols, *, dt, nt, model=None, ghost_lr=None, pulse_mode=L.PULSE_KNOTS,
                  n_amp=0, alpha=0.5, gamma=(1.0, 0.2, 0.2), S=None, s_diag=0.0, precision=L.F32, ckpt=50,
-                 fwd_tile=0, adj_tile=0):
+                 fwd_tile=0, adj_tile=0, lazy_workspace=False):
         self._keep = []
         ai = np.ascontiguousarray(np.ravel(active_index), dtype=np.int32)
         p = L.Problem()
@@ -83,6 +83,7 @@ class Instance:
             p.S_val = L._d(val)
         p.s_diag = float(s_diag)
         p.precision, p.ckpt, p.fwd_tile, p.adj_tile = int(precision), int(ckpt), int(fwd_tile), int(adj_tile)
+        p.lazy_workspace = 1 if lazy_workspace else 0
         self.problem = p
         self.nrows, self.ncols, self.nt, self.dt = p.nrows, p.ncols, p.nt, p.dt
         self.precision = p.precision
@@ -168,6 +169,65 @@ def batch_forward(instances, Es, nsteps):
     hs = (C.c_void_p * n)(*[inst.h for inst in instances])
     ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
     check(L.lib().fkadj_batch_forward(n, hs, ep, int(nsteps)), "fkadj_batch_forward")
+
+
+class Server:
+    """Continuous-batching gradient server (C ABI ``fkadj_server_*``)."""
+
+    def __init__(self, budget_bytes: int, max_slots: int = 64):
+        h = C.c_void_p()
+        check(L.lib().fkadj_server_create(int(budget_bytes), int(max_slots), C.byref(h)), "fkadj_server_create")
+        self.h = h
+        self._n = {}
+
+    def submit(self, inst: Instance, E) -> int:
+        E = L._f64(np.ravel(E))
+        if E.size != inst.n_controls:
+            raise ValueError(f"E must have {inst.n_controls} values")
+        rid = C.c_int64()
+        check(L.lib().fkadj_server_submit(self.h, inst.h, L._d(E), C.byref(rid)), "fkadj_server_submit")
+        self._n[rid.value] = inst.n_controls
+        return rid.value
+
+    def cancel(self, rid: int):
+        check(L.lib().fkadj_server_cancel(self.h, int(rid)), "fkadj_server_cancel")
+
+    def run(self, max_rounds: int = 1) -> int:
+        n = L.lib().fkadj_server_run(self.h, int(max_rounds))
+        if n < 0:
+            check(n, "fkadj_server_run")
+        return n
+
+    def poll(self, rid: int):
+        """None while running; (grad, Lcost, L0, LE) when done; raises FkadjError on failure."""
+        g = np.zeros(self._n.get(rid, 1))
+        Lc, L0, LE = C.c_double(), C.c_double(), C.c_double()
+        rc = L.lib().fkadj_server_poll(self.h, int(rid), L._d(g), C.byref(Lc), C.byref(L0), C.byref(LE))
+        if rc == L.ERR_BUSY:
+            return None
+        self._n.pop(rid, None)
+        check(rc, "fkadj_server_poll")
+        return g, Lc.value, L0.value, LE.value
+
+    def pending(self) -> int:
+        return int(L.lib().fkadj_server_pending(self.h))
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().fkadj_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
 
 
 def footprint_bytes(problem: L.Problem) -> int:

@@ -105,6 +105,9 @@ typedef struct fkadj_problem {
                                          0 -> built-in choice by the cells of the launch (the
                                          forward tile never changes a result) */
     int adj_tile;                     /* FP32 adjoint tile, same ids; 0 -> built-in table */
+    int lazy_workspace;               /* 1: checkpoint/segment/adjoint buffers are allocated on first use
+                                         and released after each request a server completes, so a
+                                         server's memory budget covers only admitted requests */
 } fkadj_problem;
 
 typedef struct fkadj_instance fkadj_instance;
@@ -139,7 +142,7 @@ int fkadj_forward(fkadj_instance *inst, const double *E, int nsteps, int snap_ev
 int fkadj_stimulate(fkadj_instance *inst, const uint8_t *cell_mask, double value);
 
 /* ---------------------------------------------------------------- batched evaluation */
-/* Gradients of `count` FP32 instances (any grid sizes, equal nt and ckpt) computed together:
+/* Gradients of `count` (<= 64) FP32 instances (any grid sizes, equal nt and ckpt) computed together:
  * every forward, recompute and adjoint step of all instances is one kernel launch over a
  * tile work-list of all instances.  Results equal the instances' own fkadj_grad bit for bit. */
 int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
@@ -161,6 +164,32 @@ typedef struct fkadj_timing {
 /* Enables per-launch HIP events on the next calls (adds small overhead); 0 disables. */
 int fkadj_set_profiling(int on);
 int fkadj_last_timing(fkadj_timing *t);
+
+/* ---------------------------------------------------------------- serving */
+/* Continuous-batching gradient server on one GPU (one host thread drives it).  Requests are
+ * admitted when a slot is free and their device footprint (fkadj_footprint_bytes) fits the
+ * remaining budget; admitted requests advance together, one forward-kind and one adjoint launch
+ * per round over all slots, each slot at its own step and phase; new requests join and finished
+ * ones leave at round boundaries.  Each result equals the same request evaluated alone
+ * (fkadj_grad on the instance) bit for bit.  Instances created with lazy_workspace hold their
+ * checkpoint buffers only while admitted.  No reference counterpart (the reference evaluates one
+ * gradient per synchronous MEX call). */
+typedef struct fkadj_server fkadj_server;
+
+int fkadj_server_create(int64_t budget_bytes, int max_slots /* <= 64 */, fkadj_server **out);
+int fkadj_server_destroy(fkadj_server *s);
+/* Queue a gradient request on an FP32 instance whose state is set; E is copied.  Requests on one
+ * instance run one at a time, in order; the instance must not be used by other calls meanwhile. */
+int fkadj_server_submit(fkadj_server *s, fkadj_instance *inst, const double *E, int64_t *request_id);
+/* A queued request is dropped; a running one leaves at the next round boundary (others unaffected). */
+int fkadj_server_cancel(fkadj_server *s, int64_t request_id);
+/* Issue up to max_rounds rounds (0: until at least one request completes or nothing is left).
+ * Returns the number of requests that completed during the call (>= 0) or an error code. */
+int fkadj_server_run(fkadj_server *s, int max_rounds);
+/* Result of a finished request (then forgotten): FKADJ_ERR_BUSY while queued or running,
+ * FKADJ_ERR_CANCELLED, FKADJ_ERR_NOMEM (footprint above the whole budget), ... */
+int fkadj_server_poll(fkadj_server *s, int64_t request_id, double *grad, double *Lcost, double *L0, double *LE);
+int fkadj_server_pending(const fkadj_server *s); /* queued + running + finishing requests */
 
 #ifdef __cplusplus
 }

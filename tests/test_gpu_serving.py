@@ -1,0 +1,100 @@
+"""GPU tests of the continuous-batching server: every result equals the request evaluated alone,
+admission respects the memory budget, cancellation and oversize requests fail only themselves."""
+import numpy as np
+import pytest
+
+import fkadj
+from cases import disc_case
+
+pytestmark = pytest.mark.gpu
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _instances(specs, lazy=True):
+    """specs: (H, W, nt, n_amp, ckpt, seed) -> FP32 piecewise instances with disc holes."""
+    m = fkadj.reference_model(h=0.025)
+    out = []
+    for (H, W, nt, K, ck, seed) in specs:
+        case = disc_case(H, W, 5 + seed % 4, seed)
+        inst = fkadj.Instance(case.ai, H, W, dt=0.02, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                              alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=ck,
+                              lazy_workspace=lazy)
+        inst.set_state(case.y0)
+        A = np.random.default_rng(seed).uniform(0, 10, K)
+        out.append((inst, A))
+    return out
+
+
+SPECS = [(64, 64, 60, 6, 20, 1), (96, 128, 90, 9, 25, 2), (40, 192, 40, 4, 10, 3), (80, 96, 120, 12, 50, 4),
+         (128, 64, 75, 5, 15, 5)]
+
+
+def test_mixed_requests_equal_alone():
+    _print_case("server_mix", n=len(SPECS))
+    pairs = _instances(SPECS)
+    alone = [inst.grad(A) for inst, A in pairs]
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=8) as srv:
+        ids = [srv.submit(*pairs[0]), srv.submit(*pairs[1])]
+        srv.run(37)                                  # later requests join mid-flight
+        ids += [srv.submit(*pairs[2]), srv.submit(*pairs[3])]
+        srv.run(11)
+        ids.append(srv.submit(*pairs[4]))
+        while srv.pending():
+            srv.run(0)
+        res = [srv.poll(i) for i in ids]
+    for a, r in zip(alone, res):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
+    for inst, _ in pairs:
+        inst.close()
+
+
+def test_budget_admits_one_at_a_time_and_repeats_on_one_instance():
+    _print_case("server_budget")
+    pairs = _instances(SPECS[:3])
+    alone = [inst.grad(A) for inst, A in pairs]
+    biggest = max(fkadj.footprint_bytes(inst.problem) for inst, _ in pairs)
+    with fkadj.Server(budget_bytes=biggest, max_slots=4) as srv:
+        ids = [srv.submit(inst, A) for inst, A in pairs]
+        ids.append(srv.submit(*pairs[0]))            # the same instance again: runs after the first
+        while srv.pending():
+            srv.run(0)
+        res = [srv.poll(i) for i in ids]
+    for a, r in zip(alone + [alone[0]], res):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1]
+    for inst, _ in pairs:
+        inst.close()
+
+
+def test_oversize_and_cancel_fail_only_themselves():
+    _print_case("server_cancel")
+    pairs = _instances(SPECS[:3])
+    alone = [inst.grad(A) for inst, A in pairs]
+    small = min(fkadj.footprint_bytes(inst.problem) for inst, _ in pairs)
+    big = max(range(3), key=lambda k: fkadj.footprint_bytes(pairs[k][0].problem))
+    with fkadj.Server(budget_bytes=fkadj.footprint_bytes(pairs[big][0].problem), max_slots=4) as srv:
+        ids = [srv.submit(inst, A) for inst, A in pairs]
+        srv.run(5)
+        victim = [k for k in range(3) if k != big][0]
+        srv.cancel(ids[victim])
+        while srv.pending():
+            srv.run(0)
+        for k in range(3):
+            if k == victim:
+                with pytest.raises(fkadj.FkadjError) as e:
+                    srv.poll(ids[k])
+                assert e.value.code == -7
+            else:
+                r = srv.poll(ids[k])
+                assert np.array_equal(alone[k][0], r[0])
+    with fkadj.Server(budget_bytes=small // 2, max_slots=2) as srv:
+        rid = srv.submit(*pairs[0])
+        while srv.pending():
+            srv.run(0)
+        with pytest.raises(fkadj.FkadjError) as e:
+            srv.poll(rid)
+        assert e.value.code == -3
+    for inst, _ in pairs:
+        inst.close()

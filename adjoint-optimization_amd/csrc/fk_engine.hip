@@ -104,6 +104,11 @@ bool all_finite(const double *a, size_t n)
 }  // namespace
 
 // ============================================================================ instance
+// An instance is the resident problem: geometry codes, objective matrix, model constants and the
+// initial state.  Everything one gradient evaluation writes lives in an EvalCtx, so several
+// evaluations of one instance (a server's requests) can be in flight at once.
+struct EvalCtx;
+
 struct fkadj_instance {
     fkadj_problem p{};
     int rows = 0, cols = 0, pitch = 0, n = 0, n_num = 0;
@@ -115,9 +120,8 @@ struct fkadj_instance {
     Model32 m32{};
     std::vector<int32_t> cell_of;
     bool state_set = false;
-    bool ws_ready = false;   // checkpoint / segment / adjoint workspace allocated
-    bool lazy_ws = false;    // workspace is allocated on use and released after a served request
-    int busy = 0;            // requests of a server in flight on this instance
+    bool lazy_ws = false;    // the default evaluation context is allocated on first use
+    int busy = 0;            // server requests in flight on this instance
 
     uint16_t *d_code = nullptr;
     int32_t *d_cell_of = nullptr;
@@ -125,30 +129,36 @@ struct fkadj_instance {
     int32_t *d_colind = nullptr;
     double *d_val = nullptr;
     double *d_gamma = nullptr;
-    char *d_y0 = nullptr, *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
-    char *d_kseg = nullptr, *d_kT = nullptr;   // FP32: stored slopes of the recompute sweep
-    double *d_Ef = nullptr, *d_Ea = nullptr;
-    double *d_part = nullptr, *d_partT = nullptr, *d_zgh = nullptr, *d_L0part = nullptr, *d_L0 = nullptr;
+    char *d_y0 = nullptr;
     double *d_packed = nullptr;
     uint8_t *d_mask = nullptr;
     hipStream_t stream = nullptr;
     int nb_plain = 0;     // plain blocks (zgh partial row length of the plain path)
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
-
-    // fused path
     int fwd_cfg = 0, adj_cfg = TILE_16_512;   // fused tile configurations (fwd 0: per group)
-    FusedSlot slot_h{};
-    struct FusedGroupDev *group = nullptr;   // this instance alone, for fkadj_grad / fkadj_forward
 
-    std::vector<double> Ef, Ea;   // host tables of the last evaluation
+    EvalCtx *ctx = nullptr;   // default context of fkadj_grad / fkadj_forward / batches
 
     char *state(char *base, int idx) const { return base + (size_t)idx * st_bytes; }
 };
 
-// A set of FP32 instances advanced together: one forward launch per tile height and one adjoint
-// launch per time step cover all members.  A single instance is a group of one.
+struct EvalCtx {
+    fkadj_instance *I = nullptr;
+    double *d_Ef = nullptr, *d_Ea = nullptr;          // stage-time field tables, 3 per step
+    double *d_part = nullptr, *d_partT = nullptr;     // z.gh partials: adjoint steps, final time
+    double *d_zgh = nullptr, *d_L0part = nullptr, *d_L0 = nullptr;
+    char *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
+    char *d_kseg = nullptr, *d_kT = nullptr;          // FP32: stored slopes of the recompute sweep
+    FusedSlot slot_h{};
+    struct FusedGroupDev *group = nullptr;            // this context alone (FP32)
+    std::vector<double> Ef, Ea;                       // host tables of the last evaluation
+    int64_t bytes = 0;
+};
+
+// A set of FP32 evaluations advanced together: one forward launch per tile configuration and one
+// adjoint launch per time step cover all members.  A single evaluation is a group of one.
 struct FusedGroupDev {
-    std::vector<fkadj_instance *> members;
+    std::vector<EvalCtx *> members;
     FusedSlot *d_slots = nullptr;
     // one tile list per kernel configuration (index = tile config id)
     TileTask *d_fwd[NUM_TILE_CFG] = {}, *d_adj[NUM_TILE_CFG] = {};
@@ -160,10 +170,8 @@ namespace {
 template <typename T>
 T *as(char *p) { return reinterpret_cast<T *>(p); }
 
-int ensure_workspace(fkadj_instance *I);
-void release_workspace(fkadj_instance *I);
 void group_free(FusedGroupDev *g);
-int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, FusedGroupDev **out);
+int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out);
 
 void fill_models(fkadj_instance *I)
 {
@@ -292,15 +300,15 @@ int validate_and_codes(const fkadj_problem *p, int pitch, std::vector<uint16_t> 
 
 template <typename T>
 PlainCtx<T, typename std::conditional<std::is_same<T, double>::value, Model64, Model32>::type>
-plain_ctx(fkadj_instance *I)
+plain_ctx(fkadj_instance *I, EvalCtx *C)
 {
     PlainCtx<T, typename std::conditional<std::is_same<T, double>::value, Model64, Model32>::type> c;
     c.g = GridArgs{I->rows, I->pitch, I->n, I->d_code};
     if constexpr (std::is_same<T, double>::value) c.m = I->m64; else c.m = I->m32;
     c.dt = I->p.dt;
-    c.Ef = I->d_Ef;
-    c.Ea = I->d_Ea;
-    T *w = as<T>(I->d_work);
+    c.Ef = C->d_Ef;
+    c.Ea = C->d_Ea;
+    T *w = as<T>(C->d_work);
     size_t s = (size_t)3 * I->n;
     c.w0 = w; c.w1 = w + s; c.w2 = w + 2 * s; c.w3 = w + 3 * s; c.w4 = w + 4 * s; c.w5 = w + 5 * s;
     c.stream = I->stream;
@@ -308,42 +316,42 @@ plain_ctx(fkadj_instance *I)
 }
 
 // Location of forward state `step` during the forward sweep (checkpoint or ring buffer).
-char *fwd_loc(fkadj_instance *I, int step)
+char *fwd_loc(fkadj_instance *I, EvalCtx *C, int step)
 {
-    if (step % I->K == 0 || step == I->nt) return I->state(I->d_ck, (step + I->K - 1) / I->K);
-    return I->state(I->d_ring, step & 1);
+    if (step % I->K == 0 || step == I->nt) return I->state(C->d_ck, (step + I->K - 1) / I->K);
+    return I->state(C->d_ring, step & 1);
 }
 
-// Stage-time field tables (see fkadj.h for the two pulse modes).
-int build_tables(fkadj_instance *I, const double *E)
+// Stage-time field tables (see fkadj.h for the two pulse modes), uploaded on stream s.
+int build_tables(fkadj_instance *I, EvalCtx *C, const double *E, hipStream_t s)
 {
     const int nt = I->nt;
-    I->Ef.assign((size_t)3 * nt, 0.0);
-    I->Ea.assign((size_t)3 * nt, 0.0);
+    C->Ef.assign((size_t)3 * nt, 0.0);
+    C->Ea.assign((size_t)3 * nt, 0.0);
     if (I->p.pulse_mode == FKADJ_PULSE_KNOTS) {
         std::vector<double> t(nt + 1), ypp;
         for (int i = 0; i <= nt; i++) t[i] = i * I->p.dt;
         if (!spline_not_a_knot(nt + 1, t.data(), E, ypp)) return FKADJ_ERR_ARG;
         for (int i = 0; i < nt; i++) {
-            I->Ef[3 * i] = E[i];
-            I->Ef[3 * i + 1] = spline_eval(t.data(), E, ypp, i, (i + 0.5) * I->p.dt);        // G:1692-1694
-            I->Ef[3 * i + 2] = E[i + 1];
+            C->Ef[3 * i] = E[i];
+            C->Ef[3 * i + 1] = spline_eval(t.data(), E, ypp, i, (i + 0.5) * I->p.dt);        // G:1692-1694
+            C->Ef[3 * i + 2] = E[i + 1];
         }
         for (int i = 1; i <= nt; i++) {
-            I->Ea[3 * (i - 1)] = E[i - 1];
-            I->Ea[3 * (i - 1) + 1] = E[i];
-            I->Ea[3 * (i - 1) + 2] = spline_eval(t.data(), E, ypp, i - 1, (i - 1 + 1.0 / 3.0) * I->p.dt);  // G:1877
+            C->Ea[3 * (i - 1)] = E[i - 1];
+            C->Ea[3 * (i - 1) + 1] = E[i];
+            C->Ea[3 * (i - 1) + 2] = spline_eval(t.data(), E, ypp, i - 1, (i - 1 + 1.0 / 3.0) * I->p.dt);  // G:1877
         }
     } else {
         const int P = nt / I->p.n_amp;
         for (int i = 0; i < nt; i++) {
             double a = E[i / P];
-            I->Ef[3 * i] = I->Ef[3 * i + 1] = I->Ef[3 * i + 2] = a;
-            I->Ea[3 * i] = I->Ea[3 * i + 1] = I->Ea[3 * i + 2] = a;
+            C->Ef[3 * i] = C->Ef[3 * i + 1] = C->Ef[3 * i + 2] = a;
+            C->Ea[3 * i] = C->Ea[3 * i + 1] = C->Ea[3 * i + 2] = a;
         }
     }
-    HIPCHK(hipMemcpyAsync(I->d_Ef, I->Ef.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, I->stream));
-    HIPCHK(hipMemcpyAsync(I->d_Ea, I->Ea.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, I->stream));
+    HIPCHK(hipMemcpyAsync(C->d_Ef, C->Ef.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, s));
+    HIPCHK(hipMemcpyAsync(C->d_Ea, C->Ea.data(), sizeof(double) * 3 * nt, hipMemcpyHostToDevice, s));
     return FKADJ_OK;
 }
 
@@ -445,104 +453,130 @@ struct Sampler {
     }
 };
 
-// ------------------------------------------------------------------------ workspace
-// Checkpoints, segment states, forward ring, adjoint ping-pong and (FP32) stored slopes: the
-// memory a gradient evaluation needs beyond the resident problem.  Zeroed on allocation so hole
-// cells read 0 everywhere.
-int ensure_workspace(fkadj_instance *I)
+// ------------------------------------------------------------------------ evaluation contexts
+// Device bytes of one evaluation context (tables, partials, checkpoints, segment states, ring,
+// adjoint ping-pong, FP64 stage scratch or FP32 stored slopes).
+int64_t ctx_bytes(const fkadj_instance *I)
 {
-    if (I->ws_ready) return FKADJ_OK;
+    const int64_t st = (int64_t)I->st_bytes;
+    int64_t states = I->nck + (I->K + 1) + 2 + 2 + (I->prec == FKADJ_F64 ? 6 : 2 * I->K + 1);
+    int64_t tables = 8 * (int64_t)(6 * I->nt + (int64_t)I->nt * I->nb_part + I->nb_plain + (I->nt + 1) +
+                                   (I->n_num + 255) / 256 + 1);
+    return states * st + tables;
+}
+
+void ctx_free(EvalCtx *C)
+{
+    if (!C) return;
+    for (char **q : {&C->d_ck, &C->d_seg, &C->d_ring, &C->d_z, &C->d_work, &C->d_kseg, &C->d_kT})
+        if (*q) (void)hipFree(*q);
+    for (double **q : {&C->d_Ef, &C->d_Ea, &C->d_part, &C->d_partT, &C->d_zgh, &C->d_L0part, &C->d_L0})
+        if (*q) (void)hipFree(*q);
+    group_free(C->group);
+    delete C;
+}
+
+// Allocates a zeroed context on stream s (hole cells read 0 in every state buffer).
+int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out)
+{
+    auto *C = new (std::nothrow) EvalCtx();
+    if (!C) return FKADJ_ERR_NOMEM;
+    C->I = I;
     const size_t st = I->st_bytes;
-    auto al = [&](char *&p, size_t bytes) -> bool {
-        if (hipMalloc((void **)&p, bytes) != hipSuccess) { p = nullptr; return false; }
-        return hipMemsetAsync(p, 0, bytes, I->stream) == hipSuccess;
+    auto alc = [&](char *&q, size_t bytes) -> bool {
+        if (hipMalloc((void **)&q, bytes) != hipSuccess) { q = nullptr; return false; }
+        return hipMemsetAsync(q, 0, bytes, s) == hipSuccess;
     };
-    bool ok = al(I->d_ck, st * I->nck) && al(I->d_seg, st * (I->K + 1)) && al(I->d_ring, st * 2) && al(I->d_z, st * 2);
-    if (ok && I->prec == FKADJ_F64) ok = al(I->d_work, st * 6);
-    if (ok && I->prec == FKADJ_F32) ok = al(I->d_kseg, st * 2 * I->K) && al(I->d_kT, st);
+    auto ald = [&](double *&q, size_t count) -> bool {
+        return hipMalloc((void **)&q, sizeof(double) * std::max<size_t>(count, 1)) == hipSuccess;
+    };
+    const int nt = I->nt;
+    bool ok = ald(C->d_Ef, 3 * (size_t)nt) && ald(C->d_Ea, 3 * (size_t)nt) && ald(C->d_part, (size_t)nt * I->nb_part) &&
+              ald(C->d_partT, I->nb_plain) && ald(C->d_zgh, nt + 1) && ald(C->d_L0part, objective_blocks(I->n_num)) &&
+              ald(C->d_L0, 1) && alc(C->d_ck, st * I->nck) && alc(C->d_seg, st * (I->K + 1)) &&
+              alc(C->d_ring, st * 2) && alc(C->d_z, st * 2);
+    if (ok && I->prec == FKADJ_F64) ok = alc(C->d_work, st * 6);
+    if (ok && I->prec == FKADJ_F32) ok = alc(C->d_kseg, st * 2 * I->K) && alc(C->d_kT, st);
     if (ok && I->prec == FKADJ_F32) {
-        FusedSlot &S = I->slot_h;
+        FusedSlot &S = C->slot_h;
         S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
         S.nt = I->nt; S.K = I->K; S.nck = I->nck;
         S.ntiles = I->nb_part;
         S.code = I->d_code;
-        S.ck = as<float>(I->d_ck); S.seg = as<float>(I->d_seg); S.ring = as<float>(I->d_ring); S.zb = as<float>(I->d_z);
-        S.kseg = as<float>(I->d_kseg); S.kT = as<float>(I->d_kT);
-        S.Ef = I->d_Ef; S.Ea = I->d_Ea; S.partial = I->d_part;
+        S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
+        S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT);
+        S.Ef = C->d_Ef; S.Ea = C->d_Ea; S.partial = C->d_part;
         S.m = I->m32; S.dt = (float)I->p.dt;
-        std::vector<fkadj_instance *> self{I};
-        ok = group_build(self, I->stream, &I->group) == FKADJ_OK;
+        std::vector<EvalCtx *> self{C};
+        ok = group_build(self, s, &C->group) == FKADJ_OK;
     }
-    ok = ok && hipStreamSynchronize(I->stream) == hipSuccess;
+    ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) {
-        release_workspace(I);
+        ctx_free(C);
         return FKADJ_ERR_NOMEM;
     }
-    I->ws_ready = true;
+    C->bytes = ctx_bytes(I);
+    *out = C;
     return FKADJ_OK;
 }
 
-void release_workspace(fkadj_instance *I)
+int ensure_default_ctx(fkadj_instance *I)
 {
-    if (I->stream) (void)hipStreamSynchronize(I->stream);
-    for (char **p : {&I->d_ck, &I->d_seg, &I->d_ring, &I->d_z, &I->d_work, &I->d_kseg, &I->d_kT}) {
-        if (*p) (void)hipFree(*p);
-        *p = nullptr;
-    }
-    group_free(I->group);
-    I->group = nullptr;
-    I->ws_ready = false;
+    if (I->ctx) return FKADJ_OK;
+    return ctx_create(I, I->stream, &I->ctx);
 }
 
 // ------------------------------------------------------------------------ one evaluation
 // Forward sweep with checkpoints, objective and terminal adjoint, then per segment (last to
 // first) recompute + adjoint steps.  All launches go to one stream.
 template <typename T>
-void terminal(fkadj_instance *I, hipStream_t s)
+void terminal(EvalCtx *C, hipStream_t s)
 {
-    auto pc = plain_ctx<T>(I);
+    fkadj_instance *I = C->I;
+    auto pc = plain_ctx<T>(I, C);
     pc.stream = s;
-    T *yT = as<T>(I->state(I->d_ck, I->nck - 1));
-    T *z0 = as<T>(I->state(I->d_z, 0));
+    T *yT = as<T>(I->state(C->d_ck, I->nck - 1));
+    T *z0 = as<T>(I->state(C->d_z, 0));
     objective_launch<T>(I->n_num, I->d_rowptr, I->d_colind, I->d_val, I->p.s_diag, I->d_cell_of, I->n, yT,
-                        I->d_gamma, z0, I->d_L0part, s);
-    sum_rows_launch(I->d_L0part, objective_blocks(I->n_num), 1, I->d_L0, s);
-    plain_zgh(pc, z0, I->d_partT);
-    sum_rows_launch(I->d_partT, I->nb_plain, 1, I->d_zgh + I->nt, s);
+                        I->d_gamma, z0, C->d_L0part, s);
+    sum_rows_launch(C->d_L0part, objective_blocks(I->n_num), 1, C->d_L0, s);
+    plain_zgh(pc, z0, C->d_partT);
+    sum_rows_launch(C->d_partT, I->nb_plain, 1, C->d_zgh + I->nt, s);
 }
 
-int run_plain(fkadj_instance *I, Sampler &smp)
+int run_plain(EvalCtx *C, Sampler &smp)
 {
     using T = double;
+    fkadj_instance *I = C->I;
     const int nt = I->nt, K = I->K;
     hipStream_t s = I->stream;
-    auto pc = plain_ctx<T>(I);
-    HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    auto pc = plain_ctx<T>(I, C);
+    HIPCHK(hipMemcpyAsync(I->state(C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
     for (int i = 0; i < nt; i++) {
-        T *yin = as<T>(fwd_loc(I, i)), *yout = as<T>(fwd_loc(I, i + 1));
+        T *yin = as<T>(fwd_loc(I, C, i)), *yout = as<T>(fwd_loc(I, C, i + 1));
         smp.fwd_launch(s, [&] { plain_forward_step(pc, yin, yout, i); });
     }
     HIPCHK(hipGetLastError());
-    terminal<T>(I, s);
+    terminal<T>(C, s);
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt);
         for (int j = 0; j + 1 < b - a; j++) {
-            T *yin = as<T>(j == 0 ? I->state(I->d_ck, sg) : I->state(I->d_seg, j));
-            T *yout = as<T>(I->state(I->d_seg, j + 1));
+            T *yin = as<T>(j == 0 ? I->state(C->d_ck, sg) : I->state(C->d_seg, j));
+            T *yout = as<T>(I->state(C->d_seg, j + 1));
             smp.fwd_launch(s, [&] { plain_forward_step(pc, yin, yout, a + j); });
         }
         for (int i = b; i > a; i--) {
-            T *yA = as<T>(i - 1 == a ? I->state(I->d_ck, sg) : I->state(I->d_seg, i - 1 - a));
-            T *yB = as<T>(i == b ? I->state(I->d_ck, sg + 1) : I->state(I->d_seg, i - a));
-            T *zin = as<T>(I->state(I->d_z, (nt - i) & 1));
-            T *zout = as<T>(I->state(I->d_z, (nt - i + 1) & 1));
-            double *prow = I->d_part + (size_t)(i - 1) * I->nb_part;
+            T *yA = as<T>(i - 1 == a ? I->state(C->d_ck, sg) : I->state(C->d_seg, i - 1 - a));
+            T *yB = as<T>(i == b ? I->state(C->d_ck, sg + 1) : I->state(C->d_seg, i - a));
+            T *zin = as<T>(I->state(C->d_z, (nt - i) & 1));
+            T *zout = as<T>(I->state(C->d_z, (nt - i + 1) & 1));
+            double *prow = C->d_part + (size_t)(i - 1) * I->nb_part;
             smp.adj_launch(s, [&] { plain_adjoint_step(pc, yA, yB, zin, zout, i - 1, prow); });
         }
         HIPCHK(hipGetLastError());
     }
-    sum_rows_launch(I->d_part, I->nb_part, nt, I->d_zgh, s);
+    sum_rows_launch(C->d_part, I->nb_part, nt, C->d_zgh, s);
     HIPCHK(hipGetLastError());
     return FKADJ_OK;
 }
@@ -558,7 +592,7 @@ void group_free(FusedGroupDev *g)
     delete g;
 }
 
-int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, FusedGroupDev **out)
+int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
 {
     if (members.empty() || members.size() > (size_t)MAX_SLOTS) return FKADJ_ERR_ARG;
     auto *g = new (std::nothrow) FusedGroupDev();
@@ -567,10 +601,10 @@ int group_build(const std::vector<fkadj_instance *> &members, hipStream_t s, Fus
     std::vector<FusedSlot> slots;
     std::vector<TileTask> fl[NUM_TILE_CFG], al[NUM_TILE_CFG];
     int64_t cells = 0;
-    for (auto *I : members) cells += (int64_t)I->rows * I->pitch;
+    for (auto *C : members) cells += (int64_t)C->I->rows * C->I->pitch;
     for (size_t k = 0; k < members.size(); k++) {
-        fkadj_instance *I = members[k];
-        slots.push_back(I->slot_h);
+        fkadj_instance *I = members[k]->I;
+        slots.push_back(members[k]->slot_h);
         const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
         auto ft = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(fc));
         fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
@@ -618,24 +652,23 @@ void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 
 int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
 {
-    fkadj_instance *I0 = g->members[0];
+    fkadj_instance *I0 = g->members[0]->I;
     const int nt = I0->nt, K = I0->K;
-    for (auto *I : g->members)
-        HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    for (auto *C : g->members)
+        HIPCHK(hipMemcpyAsync(C->I->state(C->d_ck, 0), C->I->d_y0, C->I->st_bytes, hipMemcpyDeviceToDevice, s));
     for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
     HIPCHK(hipGetLastError());
-    for (auto *I : g->members) terminal<float>(I, s);
+    for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt);
         // all b-a steps are replayed: the last one only to store the slope f(y_{b-1})
         for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
-        for (int i = b; i > a; i--)
-            smp.adj_launch(s, [&] { group_adjoint(g, i, a, s); });
+        for (int i = b; i > a; i--) smp.adj_launch(s, [&] { group_adjoint(g, i, a, s); });
         HIPCHK(hipGetLastError());
     }
-    for (auto *I : g->members) sum_rows_launch(I->d_part, I->nb_part, nt, I->d_zgh, s);
+    for (auto *C : g->members) sum_rows_launch(C->d_part, C->I->nb_part, nt, C->d_zgh, s);
     HIPCHK(hipGetLastError());
     return FKADJ_OK;
 }
@@ -653,13 +686,14 @@ void record_timing(Sampler &smp, hipEvent_t e0, hipEvent_t e1, int64_t cell_step
     g_timing.cell_steps = cell_steps;
 }
 
-int fetch_and_assemble(fkadj_instance *I, hipStream_t s, const double *E, double *grad, double *Lc, double *L0,
+int fetch_and_assemble(EvalCtx *C, hipStream_t s, const double *E, double *grad, double *Lc, double *L0,
                        double *LE)
 {
+    fkadj_instance *I = C->I;
     std::vector<double> zgh(I->nt + 1);
     double L0v = 0.0;
-    HIPCHK(hipMemcpyAsync(zgh.data(), I->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&L0v, I->d_L0, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(zgh.data(), C->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&L0v, C->d_L0, sizeof(double), hipMemcpyDeviceToHost, s));
     HIPCHK(hipStreamSynchronize(s));
     if (!all_finite(zgh.data(), zgh.size()) || !std::isfinite(L0v)) return FKADJ_ERR_NONFINITE;
     assemble(I, E, zgh.data(), L0v, grad, Lc, L0, LE);
@@ -669,17 +703,17 @@ int fetch_and_assemble(fkadj_instance *I, hipStream_t s, const double *E, double
 int eval_one(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
 {
     if (!I->state_set) return FKADJ_ERR_STATE;
-    if (I->busy) return FKADJ_ERR_BUSY;
     if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
-    CHK(ensure_workspace(I));
-    CHK(build_tables(I, E));
+    CHK(ensure_default_ctx(I));
+    EvalCtx *C = I->ctx;
+    CHK(build_tables(I, C, E, I->stream));
     Sampler smp(g_profiling.load() != 0);
     hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
     HIPCHK(hipEventRecord(e0, I->stream));
-    int rc = (I->prec == FKADJ_F64) ? run_plain(I, smp) : run_fused(I->group, I->stream, smp);
+    int rc = (I->prec == FKADJ_F64) ? run_plain(C, smp) : run_fused(C->group, I->stream, smp);
     if (rc) return rc;
     HIPCHK(hipEventRecord(e1, I->stream));
-    CHK(fetch_and_assemble(I, I->stream, E, grad, Lc, L0, LE));
+    CHK(fetch_and_assemble(C, I->stream, E, grad, Lc, L0, LE));
     record_timing(smp, e0, e1, (int64_t)I->rows * I->pitch * I->nt);
     return FKADJ_OK;
 }
@@ -844,7 +878,6 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
         return FKADJ_ERR_DEVICE;
     }
     const size_t st = I->st_bytes;
-    const int nt = I->nt;
     I->nb_plain = (I->pitch / PLAIN_BX) * ((I->rows + PLAIN_BY - 1) / PLAIN_BY);
     if (I->prec == FKADJ_F32) {
         I->fwd_cfg = p->fwd_tile;   // 0: chosen per group from its total size
@@ -858,13 +891,6 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     DMALLOC(I->d_cell_of, sizeof(int32_t) * I->n_num);
     DMALLOC(I->d_gamma, sizeof(double) * 3);
     DMALLOC(I->d_y0, st);
-    DMALLOC(I->d_Ef, sizeof(double) * 3 * nt);
-    DMALLOC(I->d_Ea, sizeof(double) * 3 * nt);
-    DMALLOC(I->d_part, sizeof(double) * (size_t)nt * I->nb_part);
-    DMALLOC(I->d_partT, sizeof(double) * I->nb_plain);
-    DMALLOC(I->d_zgh, sizeof(double) * (nt + 1));
-    DMALLOC(I->d_L0part, sizeof(double) * objective_blocks(I->n_num));
-    DMALLOC(I->d_L0, sizeof(double));
     DMALLOC(I->d_packed, sizeof(double) * 3 * I->n_num);
     DMALLOC(I->d_mask, I->n);
     if (!rowptr.empty()) {
@@ -884,7 +910,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     if (!ok) return fail(FKADJ_ERR_DEVICE);
     I->lazy_ws = p->lazy_workspace != 0;
     if (!I->lazy_ws) {
-        int wrc = ensure_workspace(I);
+        int wrc = ensure_default_ctx(I);
         if (wrc) return fail(wrc);
     }
 #undef DMALLOC
@@ -899,10 +925,9 @@ int fkadj_destroy(fkadj_instance *I)
     if (I->busy) return FKADJ_ERR_BUSY;
     if (I->stream) (void)hipStreamSynchronize(I->stream);
     dfree(I->d_code); dfree(I->d_cell_of); dfree(I->d_rowptr); dfree(I->d_colind); dfree(I->d_val); dfree(I->d_gamma);
-    release_workspace(I);
-    dfree(I->d_y0);
-    dfree(I->d_Ef); dfree(I->d_Ea); dfree(I->d_part); dfree(I->d_partT); dfree(I->d_zgh); dfree(I->d_L0part);
-    dfree(I->d_L0); dfree(I->d_packed); dfree(I->d_mask);
+    ctx_free(I->ctx);
+    I->ctx = nullptr;
+    dfree(I->d_y0); dfree(I->d_packed); dfree(I->d_mask);
     if (I->stream) (void)hipStreamDestroy(I->stream);
     delete I;
     return FKADJ_OK;
@@ -914,6 +939,7 @@ int fkadj_n_controls(const fkadj_instance *I) { return I ? I->ncontrols : FKADJ_
 int fkadj_set_state(fkadj_instance *I, const double *y)
 {
     if (!I || !y) return FKADJ_ERR_ARG;
+    if (I->busy) return FKADJ_ERR_BUSY;   // served requests still read the resident state
     if (!all_finite(y, (size_t)3 * I->n_num)) return FKADJ_ERR_NONFINITE;
     HIPCHK(hipMemcpyAsync(I->d_packed, y, sizeof(double) * 3 * I->n_num, hipMemcpyHostToDevice, I->stream));
     if (I->prec == FKADJ_F64)
@@ -959,26 +985,27 @@ int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every
     if (!I->state_set) return FKADJ_ERR_STATE;
     if (I->busy) return FKADJ_ERR_BUSY;
     if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
-    CHK(ensure_workspace(I));
-    CHK(build_tables(I, E));
+    CHK(ensure_default_ctx(I));
+    EvalCtx *C = I->ctx;
     hipStream_t s = I->stream;
-    HIPCHK(hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    CHK(build_tables(I, C, E, s));
+    HIPCHK(hipMemcpyAsync(I->state(C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
     const int nsnap = snap_every > 0 ? nsteps / snap_every - 1 : 0;   // FM:7-25: final state excluded
     int taken = 0;
     for (int i = 0; i < nsteps; i++) {
         if (I->prec == FKADJ_F32) {
-            group_forward(I->group, MODE_FWD, i, 0, s);
+            group_forward(C->group, MODE_FWD, i, 0, s);
         } else {
-            auto pc = plain_ctx<double>(I);
-            plain_forward_step(pc, as<double>(fwd_loc(I, i)), as<double>(fwd_loc(I, i + 1)), i);
+            auto pc = plain_ctx<double>(I, C);
+            plain_forward_step(pc, as<double>(fwd_loc(I, C, i)), as<double>(fwd_loc(I, C, i + 1)), i);
         }
         HIPCHK(hipGetLastError());
         if (snap_every > 0 && (i + 1) % snap_every == 0 && taken < nsnap) {
-            CHK(pack_state(I, fwd_loc(I, i + 1), snaps + (size_t)taken * 3 * I->n_num));
+            CHK(pack_state(I, fwd_loc(I, C, i + 1), snaps + (size_t)taken * 3 * I->n_num));
             taken++;
         }
     }
-    char *fin = fwd_loc(I, nsteps);
+    char *fin = fwd_loc(I, C, nsteps);
     if (y_final) CHK(pack_state(I, fin, y_final));
     if (advance) HIPCHK(hipMemcpyAsync(I->d_y0, fin, I->st_bytes, hipMemcpyDeviceToDevice, s));
     HIPCHK(hipStreamSynchronize(s));
@@ -998,23 +1025,24 @@ int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *c
         if (I->busy) return FKADJ_ERR_BUSY;
         if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
     }
-    for (int k = 0; k < count; k++) CHK(ensure_workspace(insts[k]));
     hipStream_t s = insts[0]->stream;
+    std::vector<EvalCtx *> members;
     for (int k = 0; k < count; k++) {
-        CHK(build_tables(insts[k], E[k]));
-        HIPCHK(hipStreamSynchronize(insts[k]->stream));
+        CHK(ensure_default_ctx(insts[k]));
+        members.push_back(insts[k]->ctx);
+        CHK(build_tables(insts[k], insts[k]->ctx, E[k], s));
     }
-    std::vector<fkadj_instance *> members(insts, insts + count);
     FusedGroupDev *g = nullptr;
     CHK(group_build(members, s, &g));
     int rc = FKADJ_OK;
-    for (auto *I : members)
-        if (hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    for (auto *C : members)
+        if (hipMemcpyAsync(C->I->state(C->d_ck, 0), C->I->d_y0, C->I->st_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
             rc = FKADJ_ERR_DEVICE;
     for (int i = 0; i < nsteps && !rc; i++) group_forward(g, MODE_FWD, i, 0, s);
     if (!rc && hipGetLastError() != hipSuccess) rc = FKADJ_ERR_DEVICE;
-    for (auto *I : members)
-        if (!rc && hipMemcpyAsync(I->d_y0, fwd_loc(I, nsteps), I->st_bytes, hipMemcpyDeviceToDevice, s) != hipSuccess)
+    for (auto *C : members)
+        if (!rc && hipMemcpyAsync(C->I->d_y0, fwd_loc(C->I, C, nsteps), C->I->st_bytes, hipMemcpyDeviceToDevice, s) !=
+                       hipSuccess)
             rc = FKADJ_ERR_DEVICE;
     if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = FKADJ_ERR_DEVICE;
     group_free(g);
@@ -1024,6 +1052,7 @@ int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *c
 int fkadj_stimulate(fkadj_instance *I, const uint8_t *cell_mask, double value)
 {
     if (!I || !cell_mask || !std::isfinite(value)) return FKADJ_ERR_ARG;
+    if (I->busy) return FKADJ_ERR_BUSY;
     if (!I->state_set) return FKADJ_ERR_STATE;
     std::vector<uint8_t> m((size_t)I->n, 0);
     for (int r = 0; r < I->n_num; r++) {
@@ -1103,13 +1132,13 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
         if (I->busy) return FKADJ_ERR_BUSY;
         if (!all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
     }
-    for (int k = 0; k < count; k++) CHK(ensure_workspace(insts[k]));
     hipStream_t s = insts[0]->stream;
+    std::vector<EvalCtx *> members;
     for (int k = 0; k < count; k++) {
-        CHK(build_tables(insts[k], E[k]));
-        HIPCHK(hipStreamSynchronize(insts[k]->stream));
+        CHK(ensure_default_ctx(insts[k]));
+        members.push_back(insts[k]->ctx);
+        CHK(build_tables(insts[k], insts[k]->ctx, E[k], s));
     }
-    std::vector<fkadj_instance *> members(insts, insts + count);
     FusedGroupDev *g = nullptr;
     CHK(group_build(members, s, &g));
     Sampler smp(g_profiling.load() != 0);
@@ -1118,7 +1147,7 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
     if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = FKADJ_ERR_DEVICE;
     int64_t cells = 0;
     for (int k = 0; k < count && !rc; k++) {
-        rc = fetch_and_assemble(insts[k], s, E[k], grad ? grad[k] : nullptr, Lcost ? Lcost + k : nullptr,
+        rc = fetch_and_assemble(insts[k]->ctx, s, E[k], grad ? grad[k] : nullptr, Lcost ? Lcost + k : nullptr,
                                 L0 ? L0 + k : nullptr, LE ? LE + k : nullptr);
         cells += (int64_t)insts[k]->rows * insts[k]->pitch * insts[k]->nt;
     }
@@ -1148,6 +1177,7 @@ constexpr int kRoundsInFlight = 8;
 struct ServReq {
     int64_t id = 0;
     fkadj_instance *I = nullptr;
+    EvalCtx *C = nullptr;        // this request's buffers, allocated at admission
     std::vector<double> E;
     int state = RQ_QUEUED;
     int rc = FKADJ_OK;
@@ -1183,8 +1213,6 @@ struct fkadj_server {
 
 namespace {
 
-bool instance_busy(const fkadj_instance *I) { return I->busy > 0; }
-
 // Task lists and slot table for the occupied slots (after the stream has drained).
 int server_rebuild(fkadj_server *S)
 {
@@ -1198,7 +1226,7 @@ int server_rebuild(fkadj_server *S)
         ServReq *r = S->slot_req[k];
         if (!r) continue;
         fkadj_instance *I = r->I;
-        slots[k] = I->slot_h;
+        slots[k] = r->C->slot_h;
         const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
         auto ft = make_tiles(k, I->rows, I->pitch, tile_rows_of(fc));
         fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
@@ -1232,10 +1260,11 @@ int server_rebuild(fkadj_server *S)
 int server_finish(fkadj_server *S, ServReq *r, bool cancelled)
 {
     fkadj_instance *I = r->I;
+    EvalCtx *C = r->C;
     if (!cancelled) {
-        sum_rows_launch(I->d_part, I->nb_part, I->nt, I->d_zgh, S->stream);
-        HIPCHK(hipMemcpyAsync(r->h_out, I->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, S->stream));
-        HIPCHK(hipMemcpyAsync(r->h_out + I->nt + 1, I->d_L0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
+        sum_rows_launch(C->d_part, I->nb_part, I->nt, C->d_zgh, S->stream);
+        HIPCHK(hipMemcpyAsync(r->h_out, C->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, S->stream));
+        HIPCHK(hipMemcpyAsync(r->h_out + I->nt + 1, C->d_L0, sizeof(double), hipMemcpyDeviceToHost, S->stream));
     } else {
         r->rc = FKADJ_ERR_CANCELLED;
     }
@@ -1248,7 +1277,7 @@ int server_finish(fkadj_server *S, ServReq *r, bool cancelled)
     return FKADJ_OK;
 }
 
-// Completed requests: assemble results, return budget and (lazy) workspace.
+// Completed requests: assemble results, free their contexts and return their budget.
 int server_reap(fkadj_server *S, bool wait)
 {
     int n = 0;
@@ -1271,17 +1300,18 @@ int server_reap(fkadj_server *S, bool wait)
         r->state = r->rc == FKADJ_OK ? RQ_DONE : (r->rc == FKADJ_ERR_CANCELLED ? RQ_CANCELLED : RQ_FAILED);
         if (r->h_out) (void)hipHostFree(r->h_out);
         r->h_out = nullptr;
+        ctx_free(r->C);
+        r->C = nullptr;
         S->used -= r->bytes;
         I->busy--;
-        if (I->lazy_ws && !I->busy) release_workspace(I);
         S->finishing.erase(S->finishing.begin() + k);
         n++;
     }
     return n;
 }
 
-// Admit queued requests while slots and budget allow (FIFO; a request whose instance is busy
-// lets later requests of other instances go first).
+// Admit queued requests in FIFO order while slots and budget allow.  Each request gets its own
+// evaluation context, so several requests on one instance run side by side.
 int server_admit(fkadj_server *S)
 {
     for (size_t q = 0; q < S->queue.size();) {
@@ -1291,8 +1321,7 @@ int server_admit(fkadj_server *S)
         if (free_slot < 0) return FKADJ_OK;
         ServReq *r = S->queue[q];
         fkadj_instance *I = r->I;
-        if (instance_busy(I)) { q++; continue; }
-        const int64_t bytes = fkadj_footprint_bytes(&I->p);
+        const int64_t bytes = ctx_bytes(I);
         if (bytes > S->budget) {   // can never fit: fail it instead of blocking the queue
             r->rc = FKADJ_ERR_NOMEM;
             r->state = RQ_FAILED;
@@ -1300,15 +1329,14 @@ int server_admit(fkadj_server *S)
             continue;
         }
         if (S->used + bytes > S->budget) return FKADJ_OK;
-        int rc = ensure_workspace(I);
-        if (rc == FKADJ_OK) rc = build_tables(I, r->E.data());
-        if (rc == FKADJ_OK && hipStreamSynchronize(I->stream) != hipSuccess) rc = FKADJ_ERR_DEVICE;
+        int rc = ctx_create(I, S->stream, &r->C);
+        if (rc == FKADJ_OK) rc = build_tables(I, r->C, r->E.data(), S->stream);
         if (rc == FKADJ_OK && hipHostMalloc((void **)&r->h_out, sizeof(double) * (I->nt + 2)) != hipSuccess)
             rc = FKADJ_ERR_NOMEM;
         if (rc == FKADJ_OK && hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess)
             rc = FKADJ_ERR_DEVICE;
         if (rc == FKADJ_OK &&
-            hipMemcpyAsync(I->state(I->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, S->stream) != hipSuccess)
+            hipMemcpyAsync(I->state(r->C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, S->stream) != hipSuccess)
             rc = FKADJ_ERR_DEVICE;
         S->queue.erase(S->queue.begin() + q);
         if (rc != FKADJ_OK) {
@@ -1316,7 +1344,9 @@ int server_admit(fkadj_server *S)
             r->state = RQ_FAILED;
             if (r->h_out) (void)hipHostFree(r->h_out);
             r->h_out = nullptr;
-            if (I->lazy_ws && !I->busy) release_workspace(I);
+            if (r->C) (void)hipStreamSynchronize(S->stream);
+            ctx_free(r->C);
+            r->C = nullptr;
             continue;
         }
         I->busy++;
@@ -1362,7 +1392,7 @@ int server_round(fkadj_server *S)
         const int nt = I->nt, K = I->K;
         if (r->phase == PH_FWD) {
             if (++r->step == nt) {
-                terminal<float>(I, S->stream);                  // objective, z(T), z.gh at T
+                terminal<float>(r->C, S->stream);               // objective, z(T), z.gh at T
                 rc_rhs.c[k] = SlotCursor{MODE_RHS, nt - 1, 0, 0};
                 any_rhs = true;
                 r->phase = PH_REC;
@@ -1425,6 +1455,7 @@ int fkadj_server_destroy(fkadj_server *S)
     (void)server_reap(S, true);
     for (auto &kv : S->reqs) {
         ServReq *r = kv.second;
+        ctx_free(r->C);
         if (r->h_out) (void)hipHostFree(r->h_out);
         if (r->done) (void)hipEventDestroy(r->done);
         delete r;

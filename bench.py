@@ -64,6 +64,148 @@ def cpu_baseline(mask, y0, A, nt_full, nt_sample=100):
             "seconds_per_eval": per_eval}
 
 
+def holes_for(N: int) -> int:
+    """Disc-hole count at the C2 density (80 holes on 512^2)."""
+    return max(1, int(round(80 * N * N / 512 ** 2)))
+
+
+def prepare(sizes, seeds):
+    """S1-S2 spiral states for instances of the given sizes (prepared in shared launches per size)."""
+    from fkadj import workloads as W
+    masks = [W.holes_mask(N, holes_for(N), s) for N, s in zip(sizes, seeds)]
+    states = [None] * len(sizes)
+    for N in sorted(set(sizes)):
+        idx = [k for k, n in enumerate(sizes) if n == N]
+        st = W.spiral_states([masks[k] for k in idx], [seeds[k] for k in idx])
+        for k, y in zip(idx, st):
+            states[k] = y
+    return masks, states
+
+
+def run_c3(args):
+    """C3: 64 instances with uneven grids from {256,384,512,768,1024}^2, own holes and spiral,
+    T = 150 ms, 150 amplitudes, FP32 state, FP64 reductions, all in shared launches."""
+    import fkadj
+    from fkadj import workloads as W
+    rng = np.random.default_rng(3)
+    B, T, K_amp = 64, 150.0, 150
+    sizes = [int(x) for x in rng.choice([256, 384, 512, 768, 1024], B)]
+    seeds = list(range(100, 100 + B))
+    t0 = time.perf_counter()
+    masks, states = prepare(sizes, seeds)
+    log(f"[c3] {B} states ready in {time.perf_counter() - t0:.1f} s")
+    A = [W.amplitudes(K_amp, s) for s in seeds]
+    insts = []
+    for m, y in zip(masks, states):
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50)
+        inst.set_state(y)
+        insts.append(inst)
+    for w in range(args.warmup):
+        fkadj.batch_grad(insts, A)
+    fkadj.set_profiling(True)
+    step_ms = []
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        res = fkadj.batch_grad(insts, A)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        log(f"[c3] step {k + 1}/{args.steps}: {step_ms[-1]:.0f} ms")
+    total = time.perf_counter() - t_start
+    tm = fkadj.last_timing()
+    fkadj.set_profiling(False)
+    k0 = int(np.argmin(sizes))
+    g_alone = insts[k0].grad(A[k0])
+    same = bool(np.array_equal(g_alone[0], res[k0][0]))
+    inst64 = W.make_instance(masks[k0], T, K_amp, precision=fkadj.F64, ckpt=50)
+    inst64.set_state(states[k0])
+    g64 = inst64.grad(A[k0])
+    inst64.close()
+    cells = sum(i.nrows * ((i.ncols + 63) // 64 * 64) for i in insts)
+    nt = int(round(T / W.DT_MS))
+    out = {"metric": "gradient evals/s, C3 batched uneven grids", "value": B * args.steps / total, "unit": "evals/s",
+           "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)),
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"C3: {B} instances, grids {sorted(set(sizes))} (counts "
+                                  f"{[sizes.count(n) for n in sorted(set(sizes))]}), T={T:g} ms, {K_amp} amplitudes, K=50",
+                      "cells": cells, "nt": nt},
+           "alone_equals_batched": same,
+           "rel_err_vs_f64_smallest": {"grad": float(np.linalg.norm(res[k0][0] - g64[0]) / np.linalg.norm(g64[0])),
+                                       "J": float(abs(res[k0][1] / g64[1] - 1))},
+           "kernels_ms": {"fwd": tm["fwd_kernel_ms"], "adj": tm["adj_kernel_ms"]},
+           "eval_GBs_algorithmic": algorithmic_bytes_per_cell_step(50) * cells * nt * args.steps / total / 1e9}
+    print(json.dumps(out), flush=True)
+    for inst in insts:
+        inst.close()
+
+
+def run_c4(args):
+    """C4: a Poisson stream of gradient requests (200 req/s) on instances with grids uniform over
+    multiples of 64 in [256, 1024] and T uniform in [50, 200] ms (1 ms amplitude pieces), admitted
+    by device footprint against a 200 GB budget and served by continuous batching.  The instances
+    (geometry, spiral state) stay resident; every request is a new pulse iterate of one of them."""
+    import fkadj
+    from fkadj import workloads as W
+    rng = np.random.default_rng(4)
+    P, R, rate, budget = args.c4_instances, args.c4_requests, 200.0, 200 * 10 ** 9
+    sizes = [int(x) for x in rng.choice(np.arange(256, 1025, 64), P)]
+    Ts = [int(x) for x in rng.integers(50, 201, P)]
+    seeds = list(range(200, 200 + P))
+    t0 = time.perf_counter()
+    masks, states = prepare(sizes, seeds)
+    log(f"[c4] {P} instance states ready in {time.perf_counter() - t0:.1f} s")
+    insts = []
+    for m, y, T in zip(masks, states, Ts):
+        inst = fkadj.Instance(fkadj.geometry.active_index_from_mask(m), m.shape[0], m.shape[1], dt=W.DT_MS,
+                              nt=int(round(T / W.DT_MS)), model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE,
+                              n_amp=T, alpha=2 * W.LAMBDA, gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()),
+                              precision=fkadj.F32, ckpt=50, lazy_workspace=True)
+        inst.set_state(y)
+        insts.append(inst)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    which = [int(x) for x in rng.integers(0, P, R)]
+    Es = [W.amplitudes(Ts[w], 1000 + r) for r, w in enumerate(which)]
+    # one request alone first (warm-up and the alone-vs-mix reference of request 0)
+    ref0 = insts[which[0]].grad(Es[0])
+    srv = fkadj.Server(budget_bytes=budget, max_slots=64)
+    ids, done_t, results = [None] * R, [None] * R, {}
+    nxt = 0
+    start = time.perf_counter()
+    while len(results) < R:
+        now = time.perf_counter() - start
+        while nxt < R and arrivals[nxt] <= now:
+            ids[nxt] = srv.submit(insts[which[nxt]], Es[nxt])
+            nxt += 1
+        if srv.pending() == 0:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)) if nxt < R else 0)
+            continue
+        if srv.run(1) > 0:
+            for r in range(nxt):
+                if r in results:
+                    continue
+                out = srv.poll(ids[r])
+                if out is not None:
+                    results[r] = out
+                    done_t[r] = time.perf_counter() - start
+    makespan = max(done_t)
+    srv.close()
+    lat = np.array([1e3 * (done_t[r] - arrivals[r]) for r in range(R)])
+    same = bool(np.array_equal(results[0][0], ref0[0]) and results[0][1] == ref0[1])
+    cell_steps = sum(insts[w].nrows * ((insts[w].ncols + 63) // 64 * 64) * insts[w].nt for w in which)
+    out = {"metric": "served gradient evals/s (Poisson 200 req/s) with p50/p99 latency", "value": R / makespan,
+           "unit": "evals/s", "n_gpus": 1, "steps": R, "warmup": 1, "ms_per_step": 1e3 * makespan / R,
+           "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"C4: {R} requests on {P} resident instances, grids 256..1024 step 64, "
+                                  f"T 50..200 ms, Poisson {rate:g} req/s, budget 200 GB, up to 64 slots"},
+           "offered_req_s": rate, "achieved_req_s": R / makespan, "makespan_s": makespan,
+           "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                          "min": float(lat.min()), "max": float(lat.max())},
+           "request0_equals_alone": same,
+           "eval_GBs_algorithmic": algorithmic_bytes_per_cell_step(50) * cell_steps / makespan / 1e9}
+    print(json.dumps(out), flush=True)
+    for inst in insts:
+        inst.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -75,11 +217,19 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-evals", type=int, default=4)
+    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
+                    help="c2 (default): 16 x C2 batched; c3: 64 uneven grids; c4: served Poisson stream")
+    ap.add_argument("--c4-instances", type=int, default=24)
+    ap.add_argument("--c4-requests", type=int, default=96)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched instances of this "
               "workload are independent and would shard by instance with no communication.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c3":
+        return run_c3(args)
+    if args.workload == "c4":
+        return run_c4(args)
 
     import fkadj
     from fkadj import workloads as W

@@ -51,6 +51,20 @@ def test_mixed_requests_equal_alone():
         inst.close()
 
 
+def test_requests_on_one_instance_run_side_by_side():
+    _print_case("server_same_instance", n=3)
+    (inst, A), = _instances(SPECS[1:2])
+    As = [A, A[::-1].copy(), 0.5 * A]
+    alone = [inst.grad(a) for a in As]
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=4) as srv:
+        ids = [srv.submit(inst, a) for a in As]
+        assert srv.run(0) == 3                       # admitted together, finished in the same round
+        res = [srv.poll(i) for i in ids]
+    for a, r in zip(alone, res):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
+    inst.close()
+
+
 def test_budget_admits_one_at_a_time_and_repeats_on_one_instance():
     _print_case("server_budget")
     pairs = _instances(SPECS[:3])

@@ -34,6 +34,16 @@ __device__ inline double block_sum(double v, int tid)
 }
 __device__ inline double block_sum_256(double v, int tid) { return block_sum<256>(v, tid); }
 
+// Global-memory views of pointers that reach a kernel through a struct in memory.  Such pointers
+// are generic (flat) to the compiler: flat accesses may alias LDS, so every LDS store orders them
+// and each flat load also counts against the LDS wait counter.  Casting to the global address
+// space lets loads batch across LDS traffic and use a scalar base with a 32-bit vector offset.
+#define FK_GLOBAL __attribute__((address_space(1)))
+template <typename T>
+__device__ inline const FK_GLOBAL T *gmem(const T *p) { return (const FK_GLOBAL T *)p; }
+template <typename T>
+__device__ inline FK_GLOBAL T *gmem(T *p) { return (FK_GLOBAL T *)p; }
+
 // ---- plain path context
 template <typename T, typename M>
 struct PlainCtx {

@@ -130,28 +130,43 @@ __device__ inline void jtz(const Jac32 &J, const float *z, float *o)
 }
 
 // Region (tile + 4-cell halo) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
-// first 64 cells of a row (coalesced), lanes 0..7 the last 8.  Cells outside the grid read as 0
-// with code 0 (inactive).
+// first 64 cells of a row (coalesced), lanes 0..7 the last 8 of 8 rows.  Cells outside the grid
+// read as 0 with code 0 (inactive).  All loads are issued before the first LDS store (their
+// addresses are clamped into the grid, so every load is unconditional and in bounds).
 template <class G>
 __device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
                                    float (*dst)[G::RY][G::RXP], uint16_t (*sc)[G::RXP])
 {
-    constexpr int RX = G::RX, RY = G::RY, NW = G::NT / 64;
+    constexpr int RY = G::RY, NW = G::NT / 64;
+    constexpr int NR = (RY + NW - 1) / NW;              // rows per wavefront (first 64 cells)
+    constexpr int NTL = (RY + 8 * NW - 1) / (8 * NW);   // tail passes (last 8 cells, 8 rows each)
+    constexpr int NE = NR + NTL;
+    static_assert(G::RX - 64 == 8, "tail loads assume an 8-cell halo");
     const int lane = tid & 63, wave = tid >> 6;
     const int gx0 = x0 - FUSED_HALO, gy0 = y0 - FUSED_HALO;
-    static_assert(RX - 64 == 8, "tail loads assume an 8-cell halo");
-    auto cell = [&](int ly, int lx) {
-        const int gy = gy0 + ly, gx = gx0 + lx;
-        const bool in = (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
-        const int k = gy * S.pitch + gx;
-        if (sc) sc[ly][lx] = in ? S.code[k] : (uint16_t)0;
+    const auto *g = gmem(src);
+    const auto *gc = gmem(S.code);
+    int lyv[NE], lxv[NE];
+    bool in[NE];
+    float val[NE][3];
+    uint16_t cv[NE];
 #pragma unroll
-        for (int v = 0; v < 3; v++) dst[v][ly][lx] = in ? src[v * S.n + k] : 0.0f;
-    };
-    for (int ly = wave; ly < RY; ly += NW) cell(ly, lane);                    // first 64 cells of each row
-    for (int r0 = wave * 8; r0 < RY; r0 += NW * 8) {                           // last 8 cells, 8 rows per wave
-        const int ly = r0 + (lane >> 3);
-        if (ly < RY) cell(ly, 64 + (lane & 7));
+    for (int e = 0; e < NE; e++) {
+        if (e < NR) { lyv[e] = wave + e * NW; lxv[e] = lane; }
+        else { lyv[e] = (e - NR) * 8 * NW + wave * 8 + (lane >> 3); lxv[e] = 64 + (lane & 7); }
+        const int gy = gy0 + lyv[e], gx = gx0 + lxv[e];
+        in[e] = lyv[e] < RY && (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
+        const int k = in[e] ? gy * S.pitch + gx : 0;
+        cv[e] = gc[k];
+#pragma unroll
+        for (int v = 0; v < 3; v++) val[e][v] = g[v * S.n + k];
+    }
+#pragma unroll
+    for (int e = 0; e < NE; e++) {
+        if (lyv[e] >= RY) continue;
+        if (sc) sc[lyv[e]][lxv[e]] = in[e] ? cv[e] : (uint16_t)0;
+#pragma unroll
+        for (int v = 0; v < 3; v++) dst[v][lyv[e]][lxv[e]] = in[e] ? val[e][v] : 0.0f;
     }
 }
 
@@ -237,7 +252,7 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
                 if (gy >= S.rows) continue;
                 int kk = gy * S.pitch + gx;
 #pragma unroll
-                for (int v = 0; v < 3; v++) kout[v * S.n + kk] = f[sl][v];
+                for (int v = 0; v < 3; v++) gmem(kout)[v * S.n + kk] = f[sl][v];
             }
             if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
         }
@@ -265,7 +280,7 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
         if (gy >= S.rows) continue;
         int kk = gy * S.pitch + gx;
 #pragma unroll
-        for (int v = 0; v < 3; v++) yout[v * S.n + kk] = (cc[sl] & C_ACTIVE) ? y[sl][v] + (acc[sl][v] + b4 * f[sl][v]) : 0.0f;
+        for (int v = 0; v < 3; v++) gmem(yout)[v * S.n + kk] = (cc[sl] & C_ACTIVE) ? y[sl][v] + (acc[sl][v] + b4 * f[sl][v]) : 0.0f;
     }
 }
 
@@ -305,14 +320,14 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     const int segi = seg_a / S.K;
     const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
     const float *kbuf = S.kseg + (size_t)(segi & 1) * S.K * st;
-    const float *yA = (i - 1 == seg_a) ? S.ck + (size_t)segi * st : S.seg + (size_t)(i - 1 - seg_a) * st;
-    const float *yB = (i == b_end) ? S.ck + (size_t)(segi + 1) * st : S.seg + (size_t)(i - seg_a) * st;
-    const float *kA = kbuf + (size_t)(i - 1 - seg_a) * st;
+    const auto *yA = gmem((i - 1 == seg_a) ? S.ck + (size_t)segi * st : S.seg + (size_t)(i - 1 - seg_a) * st);
+    const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : S.seg + (size_t)(i - seg_a) * st);
+    const auto *kA = gmem(kbuf + (size_t)(i - 1 - seg_a) * st);
     // f(y_i): slope of the next recomputed step, the first slope of the later segment, or f(y_nt)
-    const float *kB = (i < b_end) ? kbuf + (size_t)(i - seg_a) * st
-                                  : (b_end < S.nt ? S.kseg + (size_t)((segi + 1) & 1) * S.K * st : S.kT);
+    const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st
+                                      : (b_end < S.nt ? S.kseg + (size_t)((segi + 1) & 1) * S.K * st : S.kT));
     const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
-    float *zout = S.zb + (size_t)((S.nt - i + 1) & 1) * st;
+    auto *zout = gmem(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
     const Model32 m = S.m;
     const float dt = S.dt;
     const int row = i - 1;
@@ -321,20 +336,35 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     // of this interval (they differ at a piecewise-constant jump): f_u += (E_a - E_f) gh
     const float dEB = (i < S.nt) ? (float)(S.Ea[3 * row + 1] - S.Ef[3 * i]) : 0.0f;
 
-    load_region<G>(S, zin, t.x0, t.y0, tid, sA, sC);
-    __syncthreads();
-
     int ly[NS], lx[NS], kk[NS];
     bool own[NS], act[NS];
     uint32_t cc[NS];
     LapW tw[NS];
+    // phase-1a inputs (y_{i-1}, y_i and their slopes) are requested before the region load; the
+    // index is clamped into the grid and the values of cells outside it are never used
+    float pa[NS][3], pb[NS][3], pka[NS][3], pkb[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
         own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
+        kk[sl] = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
+        const int q = min(max(kk[sl], 0), S.n - 1);
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            pa[sl][v] = yA[v * S.n + q];
+            pb[sl][v] = yB[v * S.n + q];
+            pka[sl][v] = kA[v * S.n + q];
+            pkb[sl][v] = kB[v * S.n + q];
+        }
+    }
+
+    load_region<G>(S, zin, t.x0, t.y0, tid, sA, sC);
+    __syncthreads();
+
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
         cc[sl] = sC[ly[sl]][lx[sl]];
         act[sl] = own[sl] && (cc[sl] & C_ACTIVE);
-        kk[sl] = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
         tw[sl] = lapT_weights(cc[sl]);
     }
 
@@ -344,14 +374,13 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     for (int sl = 0; sl < NS; sl++) {
         float y3[3] = {0.0f, 0.0f, 0.0f};
         if (act[sl]) {
-            const int q = kk[sl];
             const float gh = m.ghc * (float)code_gh(cc[sl]);
 #pragma unroll
             for (int v = 0; v < 3; v++) {
-                float a = yA[v * S.n + q], b = yB[v * S.n + q];
-                float fb = kB[v * S.n + q];
+                float a = pa[sl][v], b = pb[sl][v];
+                float fb = pkb[sl][v];
                 if (v == 0) fb += dEB * gh;
-                float m0 = kA[v * S.n + q] * dt, m1 = fb * dt;
+                float m0 = pka[sl][v] * dt, m1 = fb * dt;
                 y3[v] = h3<float>(a, b, m0, m1);
                 y4[sl][v] = h4p<float>(a, b, m0, m1);
                 yb[sl][v] = b;
@@ -474,7 +503,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         }
     }
     double s = block_sum<NT>(part, tid);
-    if (tid == 0) S.partial[(size_t)row * S.ntiles + t.tile] = s;
+    if (tid == 0) gmem(S.partial)[(size_t)row * S.ntiles + t.tile] = s;
 }
 
 // ------------------------------------------------------------------------------ launchers

@@ -11,6 +11,8 @@ Initial states: an S1-S2 cross-field protocol.  S1 excites a strip at the left w
 wave (about 1.2 cells/ms, action potential about 180 ms at these constants) crosses the tissue.
 When its back has passed mid-width, S2 excites the lower half behind the back; the S2 wave's
 free end at the half-height line curls into a spiral.  The S2 time is jittered from the seed.
+With two S2 bands (rows N/8..3N/8 and 5N/8..7N/8) each band has two free ends, giving four
+interacting spirals (the large-grid configuration).
 """
 from __future__ import annotations
 
@@ -61,7 +63,18 @@ def make_instance(mask: np.ndarray, T: float, K: int, precision=L.F32, ckpt=50, 
                     adj_tile=adj_tile)
 
 
-def spiral_states(masks, seeds, chunk_ms: float = 5.0, settle_ms: float = 150.0):
+def s2_region(nrows: int, ncols: int, back_col: int, bands: int) -> np.ndarray:
+    """S2 stimulus: the lower half (one free end) or two horizontal bands (four free ends)."""
+    s2 = np.zeros((nrows, ncols), bool)
+    if bands == 1:
+        s2[nrows // 2:, :back_col] = True
+    else:
+        for lo, hi in ((nrows // 8, 3 * nrows // 8), (5 * nrows // 8, 7 * nrows // 8)):
+            s2[lo:hi, :back_col] = True
+    return s2
+
+
+def spiral_states(masks, seeds, chunk_ms: float = 5.0, settle_ms: float = 150.0, bands: int = 1):
     """S1-S2 spiral initial states (packed) for grids of one shape, prepared in shared launches."""
     nrows, ncols = masks[0].shape
     chunk = int(round(chunk_ms / DT_MS))
@@ -87,11 +100,31 @@ def spiral_states(masks, seeds, chunk_ms: float = 5.0, settle_ms: float = 150.0)
         t += chunk_ms
         for k, inst in enumerate(prep):
             if not done[k] and t >= t_s2[k]:
-                s2 = np.zeros((nrows, ncols), bool)
-                s2[nrows // 2:, :back_col[k]] = True
-                inst.stimulate(s2, 1.0)
+                inst.stimulate(s2_region(nrows, ncols, back_col[k], bands), 1.0)
                 done[k] = True
     states = [inst.get_state() for inst in prep]
     for inst in prep:
         inst.close()
     return states
+
+
+def phase_singularities(y: np.ndarray, mask: np.ndarray) -> int:
+    """Spiral cores of a packed state: clusters of 2x2 plaquettes of active cells around which the
+    phase atan2(v - 1/2, u - 1/2) winds by a full turn."""
+    from scipy import ndimage
+
+    n = int(mask.sum())
+    u = np.zeros(mask.shape)
+    v = np.zeros(mask.shape)
+    u[mask] = y[:n]
+    v[mask] = y[n:2 * n]
+    th = np.arctan2(v - 0.5, u - 0.5)
+
+    def d(a, b):
+        return np.angle(np.exp(1j * (b - a)))
+
+    a, b, c, e = th[:-1, :-1], th[:-1, 1:], th[1:, 1:], th[1:, :-1]
+    wind = np.rint((d(a, b) + d(b, c) + d(c, e) + d(e, a)) / (2 * np.pi)).astype(int)
+    ok = mask[:-1, :-1] & mask[:-1, 1:] & mask[1:, 1:] & mask[1:, :-1]
+    _, count = ndimage.label((wind != 0) & ok, structure=np.ones((3, 3)))
+    return int(count)

@@ -64,6 +64,175 @@ def cpu_baseline(mask, y0, A, nt_full, nt_sample=100):
             "seconds_per_eval": per_eval}
 
 
+def oracle_piecewise(mask, y0, A, nt):
+    """The FP64 oracle's full gradient in piecewise mode on the host (the CPU baseline leg):
+    returns (dJ/dA, Lcost, seconds)."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from fkadj import geometry as G
+    from fkadj import workloads as W
+
+    om = O.reference_model(h=W.H_CM)
+    ai = G.active_index_from_mask(mask)
+    planes = O.num2nan(y0, ai, *mask.shape)
+    gh = np.full(mask.size, np.nan)
+    gh[ai > 0] = om.Du * 2 / om.h * G.ghost_lr_from_mask(mask)
+    K = len(A)
+    P = nt // K
+    Ef = np.repeat(np.asarray(A, float), 3 * P)
+    S = G.mean_u2_S(int(mask.sum()))
+    t0 = time.perf_counter()
+    zgh, L0, _ = O.grad(om, planes, gh.reshape(mask.shape), W.DT_MS, Ef, Ef.copy(), ai, G.csc_arrays(S), (1, 0, 0),
+                        ckpt=50)
+    sec = time.perf_counter() - t0
+    alpha, Tp = 2 * W.LAMBDA, P * W.DT_MS
+    g = np.array([alpha * A[k] * Tp - W.DT_MS * np.sum(0.5 * (zgh[k * P:(k + 1) * P] + zgh[k * P + 1:(k + 1) * P + 1]))
+                  for k in range(K)])
+    LE = Tp * float(np.sum(np.square(A)))
+    return g, 0.5 * L0 + 0.5 * alpha * LE, sec
+
+
+def roofline_of(timings, cells, ckpt, nt, evals, seconds):
+    """Dominant fused kernel's algorithmic-bytes rate from the in-run HIP-event samples."""
+    fwd_ms = float(np.mean([t["fwd_kernel_ms"] for t in timings]))
+    adj_ms = float(np.mean([t["adj_kernel_ms"] for t in timings]))
+    nf = timings[-1]["fwd_launches"]
+    na = timings[-1]["adj_launches"]
+    fwd_bytes = 7 * 4 * cells
+    adj_bytes = 13 * 4 * cells
+    if adj_ms * na >= fwd_ms * nf:
+        dom, dbytes, dms = "k_fused_adj (adjoint step)", adj_bytes, adj_ms
+    else:
+        dom, dbytes, dms = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms
+    achieved = dbytes / (dms * 1e-3) / 1e9
+    eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
+    return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "kernel_ms": dms, "bytes_per_launch": dbytes,
+            "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
+            "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},
+            "eval_GBs": eval_bytes * evals / seconds / 1e9}
+
+
+def run_c1(args):
+    """C1: 128^2, 20 holes, one S1-S2 spiral, T = 100 ms (5000 steps), 100 amplitudes, FP64.  The
+    reference configuration runs on the CPU: the FP64 oracle evaluates it in full on the host,
+    beside the plain FP64 and the fused FP32 GPU paths, and all three gradients are compared."""
+    import fkadj
+    from fkadj import workloads as W
+    cfg = W.CONFIGS["C1"]
+    N, T, K_amp = cfg["N"], cfg["T"], cfg["K"]
+    nt = int(round(T / W.DT_MS))
+    seed = 1
+    m = W.holes_mask(N, cfg["holes"], seed)
+    y0 = W.spiral_states([m], [seed])[0]
+    A = W.amplitudes(K_amp, seed)
+    res = {}
+    for name, prec in (("plain_f64", fkadj.F64), ("fused_f32", fkadj.F32)):
+        inst = W.make_instance(m, T, K_amp, precision=prec, ckpt=50)
+        inst.set_state(y0)
+        for _ in range(args.warmup):
+            inst.grad(A)
+        ts = time.perf_counter()
+        for _ in range(args.steps):
+            out = inst.grad(A)
+        res[name] = (out, (time.perf_counter() - ts) / args.steps)
+        inst.close()
+        log(f"[c1] {name}: {1e3 * res[name][1]:.1f} ms/eval")
+    g_cpu, Lc_cpu, sec = oracle_piecewise(m, y0, A, nt)
+    log(f"[c1] FP64 oracle on the host: {sec:.1f} s/eval")
+    g64, g32 = res["plain_f64"][0][0], res["fused_f32"][0][0]
+    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    out = {"metric": "gradient evals/s, C1 128^2 FP64 (plain GPU path; CPU oracle beside)",
+           "value": 1.0 / res["plain_f64"][1], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
+           "warmup": args.warmup, "ms_per_step": 1e3 * res["plain_f64"][1], "higher_is_better": True,
+           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+           "config": {"workload": f"C1: {N}^2, {cfg['holes']} holes, one spiral, T={T:g} ms ({nt} steps), "
+                                  f"{K_amp} amplitudes, FP64", "phase_singularities": W.phase_singularities(y0, m)},
+           "fused_f32": {"value": 1.0 / res["fused_f32"][1], "unit": "evals/s",
+                         "rel_err_vs_f64": {"grad": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+                                            "J": float(abs(res["fused_f32"][0][1] / res["plain_f64"][0][1] - 1))}},
+           "gpu_f64_vs_cpu_oracle": {"grad": float(np.linalg.norm(g64 - g_cpu) / np.linalg.norm(g_cpu)),
+                                     "J": float(abs(res["plain_f64"][0][1] / Lc_cpu - 1))},
+           "cpu_baseline": {"value": 1.0 / sec, "unit": "evals/s", "cores": threads, "kind": "port",
+                            "sample": "FP64 oracle, the full C1 evaluation (forward, recompute and adjoint sweeps)",
+                            "seconds_per_eval": sec}}
+    print(json.dumps(out), flush=True)
+
+
+def run_c5(args):
+    """C5: one 2048^2 instance, 400 holes, four interacting spirals (two S2 bands), T = 500 ms
+    (25 000 steps), 500 amplitudes, FP32 state with FP64 reductions, uniform checkpoints every 50
+    steps; gradient error against the FP64 path of the same discretisation."""
+    import fkadj
+    from fkadj import workloads as W
+    cfg = W.CONFIGS["C5"]
+    N, T, K_amp, ckpt = cfg["N"], cfg["T"], cfg["K"], cfg["ckpt"]
+    nt = int(round(T / W.DT_MS))
+    seed = 5
+    t0 = time.perf_counter()
+    m = W.holes_mask(N, cfg["holes"], seed)
+    y0 = W.spiral_states([m], [seed], bands=2)[0]
+    ps = W.phase_singularities(y0, m)
+    log(f"[c5] four-spiral state ready in {time.perf_counter() - t0:.1f} s ({ps} phase singularities)")
+    A = W.amplitudes(K_amp, seed)
+    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt)
+    inst.set_state(y0)
+    for _ in range(args.warmup):
+        inst.grad(A)
+    fkadj.set_profiling(True)
+    timings, step_ms = [], []
+    ts0 = time.perf_counter()
+    for k in range(args.steps):
+        ts = time.perf_counter()
+        g32, Lc32, _, _ = inst.grad(A)
+        step_ms.append(1e3 * (time.perf_counter() - ts))
+        timings.append(fkadj.last_timing())
+        log(f"[c5] eval {k + 1}/{args.steps}: {step_ms[-1]:.0f} ms")
+    total = time.perf_counter() - ts0
+    fkadj.set_profiling(False)
+    fp32_bytes = fkadj.footprint_bytes(inst.problem)
+    inst.close()
+    rel, plain = None, None
+    if not args.skip_plain:
+        inst64 = W.make_instance(m, T, K_amp, precision=fkadj.F64, ckpt=ckpt)
+        inst64.set_state(y0)
+        ts = time.perf_counter()
+        g64, Lc64, _, _ = inst64.grad(A)
+        t64 = time.perf_counter() - ts
+        inst64.close()
+        plain = {"value": 1.0 / t64, "unit": "evals/s", "ms_per_eval": 1e3 * t64}
+        rel = {"grad_rel_l2": float(np.linalg.norm(g32 - g64) / np.linalg.norm(g64)),
+               "grad_rel_max": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
+               "J_rel": float(abs(Lc32 / Lc64 - 1))}
+        log(f"[c5] plain FP64 {t64:.1f} s/eval; rel err grad {rel['grad_rel_l2']:.2e}")
+    # the same state over a shorter horizon (first 100 ms, 100 amplitudes): FP32-vs-FP64 error growth
+    horizon = None
+    if not args.skip_plain:
+        Ts, Ks = 100.0, 100
+        gs = {}
+        for prec in (fkadj.F32, fkadj.F64):
+            ish = W.make_instance(m, Ts, Ks, precision=prec, ckpt=ckpt)
+            ish.set_state(y0)
+            gs[prec] = ish.grad(A[:Ks])[0]
+            ish.close()
+        horizon = {"T_500ms": rel["grad_rel_l2"],
+                   "T_100ms": float(np.linalg.norm(gs[fkadj.F32] - gs[fkadj.F64]) / np.linalg.norm(gs[fkadj.F64]))}
+        log(f"[c5] FP32 gradient error by horizon: {horizon}")
+    cells = N * ((N + 63) // 64 * 64)
+    out = {"metric": "gradient evals/s, C5 2048^2 T=500 ms (FP32, FP64 reductions)",
+           "value": args.steps / total, "unit": "evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+           "ms_per_step": float(np.mean(step_ms)), "higher_is_better": True, "scaling": "weak",
+           "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+           "config": {"workload": f"C5: {N}^2, {cfg['holes']} holes, four S1-S2 spirals, T={T:g} ms ({nt} steps), "
+                                  f"{K_amp} amplitudes, uniform checkpoints every {ckpt} steps",
+                      "phase_singularities": ps, "footprint_GB_f32": fp32_bytes / 1e9},
+           "plain_f64": plain, "rel_err_vs_f64": rel, "grad_rel_err_by_horizon": horizon,
+           "speedup_vs_plain": (args.steps / total) / plain["value"] if plain else None,
+           "roofline": roofline_of(timings, cells, ckpt, nt, args.steps, total)}
+    print(json.dumps(out), flush=True)
+
+
 def holes_for(N: int) -> int:
     """Disc-hole count at the C2 density (80 holes on 512^2)."""
     return max(1, int(round(80 * N * N / 512 ** 2)))
@@ -217,8 +386,9 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-evals", type=int, default=4)
-    ap.add_argument("--workload", choices=("c2", "c3", "c4"), default="c2",
-                    help="c2 (default): 16 x C2 batched; c3: 64 uneven grids; c4: served Poisson stream")
+    ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
+                    help="c2 (default): 16 x C2 batched; c1: 128^2 FP64 with the CPU oracle; c3: 64 uneven "
+                         "grids; c4: served Poisson stream; c5: 2048^2, 25 000 steps")
     ap.add_argument("--c4-instances", type=int, default=24)
     ap.add_argument("--c4-requests", type=int, default=96)
     args = ap.parse_args()
@@ -226,6 +396,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched instances of this "
               "workload are independent and would shard by instance with no communication.", file=sys.stderr)
         sys.exit(2)
+    if args.workload == "c1":
+        return run_c1(args)
+    if args.workload == "c5":
+        return run_c5(args)
     if args.workload == "c3":
         return run_c3(args)
     if args.workload == "c4":
@@ -301,25 +475,7 @@ def main():
 
     # roofline of the dominant kernel from the in-run HIP-event samples
     cells = sum(inst.nrows * ((inst.ncols + 63) // 64 * 64) for inst in insts)
-    fwd_ms = float(np.mean([t["fwd_kernel_ms"] for t in timings]))
-    adj_ms = float(np.mean([t["adj_kernel_ms"] for t in timings]))
-    nf = timings[-1]["fwd_launches"]
-    na = timings[-1]["adj_launches"]
-    fwd_bytes = 7 * 4 * cells
-    adj_bytes = 13 * 4 * cells
-    fwd_share, adj_share = fwd_ms * nf, adj_ms * na
-    if adj_share >= fwd_share:
-        dom, dbytes, dms = "k_fused_adj<16> (adjoint step)", adj_bytes, adj_ms
-    else:
-        dom, dbytes, dms = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms
-    achieved = dbytes / (dms * 1e-3) / 1e9
-    eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
-    roof = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
-            "kernel_ms": dms, "bytes_per_launch": dbytes,
-            "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
-            "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},
-            "eval_GBs": eval_bytes * args.steps / total / 1e9}
+    roof = roofline_of(timings, cells, ckpt, nt, args.steps, total)   # cells: all instances of a step
 
     cpu = None if args.skip_cpu else cpu_baseline(masks[0], states[0], A[0], nt)
     if cpu:

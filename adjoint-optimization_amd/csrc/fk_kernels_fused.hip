@@ -7,10 +7,17 @@
 // A 64 x TY output tile is staged in LDS with a 4-cell halo (one cell per RK stage); stage k is
 // evaluated on the tile grown by 4-k cells, so the tile needs no neighbour communication inside a
 // step and HBM sees one read of the inputs and one write of the outputs per step.
-// Work split: thread t owns the tile cells t + NT*s (one wavefront = one 64-cell row) and, in each
-// halo ring at distance d = 1..3 from the tile, the ring cell with perimeter index t.  The
-// ownership is the same in every stage, so per-cell values (state, RK increments, Jacobians) stay
-// in registers across the stages; LDS only holds the values neighbours read.
+// Work split: thread t owns the tile cells t + NT*s (one wavefront = one 64-cell row) and ring
+// cells of the halo: the cells of rings d = 1, 2, 3 around the tile are numbered in one sequence
+// (ring 1 first) and dealt out NT at a time, every other pass in reverse thread order, so each
+// wavefront carries about the same share of halo work (a ring-d cell takes part in stages 1..4-d).
+// The ownership is the same in every stage, so per-cell values (state, RK increments, Jacobians)
+// stay in registers across the stages; LDS only holds the values neighbours read.
+// Multiply-adds are fused only inside one source expression (as written), never across
+// statements: a cell's arithmetic is then the same whichever slot (tile or halo ring) computes it,
+// so results do not depend on the tile configuration.
+#pragma clang fp contract(on)
+
 #include "fk_internal.hpp"
 
 namespace fk {
@@ -21,28 +28,48 @@ struct Tile {
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
     static constexpr int RXP = RX + 1;               // odd LDS row pitch: column walks hit distinct banks
     static constexpr int TS = TX * TY / NT;          // tile cells per thread
-    static constexpr int NSLOT = TS + 3;              // + one cell in each of the 3 halo rings
     static_assert(TX * TY % NT == 0, "tile must split evenly");
-    static_assert(2 * (TX + 6) + 2 * (TY + 4) <= NT, "ring 3 must fit one cell per thread");
-    __device__ static constexpr int ring_count(int d) { return 2 * (TX + 2 * d) + 2 * (TY + 2 * d - 2); }
+    __host__ __device__ static constexpr int ring_count(int d) { return 2 * (TX + 2 * d) + 2 * (TY + 2 * d - 2); }
+    static constexpr int RC = ring_count(1) + ring_count(2) + ring_count(3);   // halo cells computed
+    static constexpr int NRS = (RC + NT - 1) / NT;                               // ring slots per thread
+    static constexpr int NSLOT = TS + NRS;
+    // ring of the r-th halo cell in the ring-1, ring-2, ring-3 sequence
+    __host__ __device__ static constexpr int depth_of(int r)
+    {
+        return r < ring_count(1) ? 1 : (r < ring_count(1) + ring_count(2) ? 2 : 3);
+    }
+    // ring slot j holds sequence numbers NT*j .. NT*(j+1)-1: its smallest ring
+    __host__ __device__ static constexpr int min_depth(int j) { return depth_of(NT * j); }
 };
 
-// Cell of slot `sl` of thread `tid` (LDS coordinates); returns false if the thread owns no
-// cell in that ring.
+// Cell of slot `sl` of thread `tid` (LDS coordinates) and the last RK stage it takes part in
+// (4 for tile cells, 4-d for a ring-d cell); returns false if the thread owns no cell in that slot.
 template <class G>
-__device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
+__device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_stage)
 {
     constexpr int TY = G::RY - 2 * G::H;
     if (sl < G::TS) {
         int lin = tid + G::NT * sl;
         ly = G::H + lin / G::TX;
         lx = G::H + lin % G::TX;
+        last_stage = 4;
         return true;
     }
-    const int d = sl - G::TS + 1;
+    const int j = sl - G::TS;
+    int e = (j & 1) ? G::NT * (j + 1) - 1 - tid : G::NT * j + tid;
+    last_stage = 0;
+    if (e >= G::RC) return false;
+    int d = 1;
+    if (e >= G::ring_count(1)) {
+        e -= G::ring_count(1);
+        d = 2;
+        if (e >= G::ring_count(2)) {
+            e -= G::ring_count(2);
+            d = 3;
+        }
+    }
+    last_stage = 4 - d;
     const int W = G::TX + 2 * d, Hh = TY + 2 * d, base = G::H - d;
-    const int e = tid;
-    if (e >= G::ring_count(d)) return false;
     if (e < W) { ly = base; lx = base + e; }
     else if (e < 2 * W) { ly = G::H + TY + d - 1; lx = base + e - W; }
     else {
@@ -53,9 +80,10 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx)
     return true;
 }
 
-// Slot sl takes part in stage k (1..4): tile cells always, ring d while d <= 4-k.
+// Some cell of slot sl may take part in stage k (1..4): tile slots always, a ring slot if its
+// smallest ring d has d <= 4-k (which of its cells do is decided per thread).
 template <class G>
-__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || (sl - G::TS + 1) <= 4 - k; }
+__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || G::min_depth(sl - G::TS) <= 4 - k; }
 
 // Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0 in
 // LDS, so the mirror Laplacian (a missing neighbour takes the opposite one's value) is
@@ -217,14 +245,14 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
     load_region<G>(S, yin, t.x0, t.y0, tid, sY, sC);
     __syncthreads();
 
-    int ly[NS], lx[NS];
+    int ly[NS], lx[NS], ls[NS];
     bool own[NS];
     uint32_t cc[NS];
     LapW lw[NS];
     float y[NS][3], acc[G::TS][3], f[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
+        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl], ls[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
         cc[sl] = sC[ly[sl]][lx[sl]];
         lw[sl] = lap_weights(m, cc[sl]);
@@ -239,7 +267,7 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
         for (int sl = 0; sl < NS; sl++) {
             if (!slot_in_stage<G>(sl, k)) continue;
             f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
-            if (own[sl] && (cc[sl] & C_ACTIVE)) {
+            if (k <= ls[sl] && (cc[sl] & C_ACTIVE)) {
                 float yc[3];
                 rhs_lds<RY, RX>(m, sY, lw[sl], ly[sl], lx[sl], E, f[sl], yc);
             }
@@ -262,7 +290,7 @@ __global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ 
             __syncthreads();
 #pragma unroll
             for (int sl = 0; sl < NS; sl++) {
-                if (!slot_in_stage<G>(sl, k) || !own[sl]) continue;
+                if (!slot_in_stage<G>(sl, k) || k > ls[sl]) continue;
 #pragma unroll
                 for (int v = 0; v < 3; v++) sY[v][ly[sl]][lx[sl]] = y[sl][v] + a * f[sl][v];
                 if (sl < G::TS) {
@@ -336,7 +364,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     // of this interval (they differ at a piecewise-constant jump): f_u += (E_a - E_f) gh
     const float dEB = (i < S.nt) ? (float)(S.Ea[3 * row + 1] - S.Ef[3 * i]) : 0.0f;
 
-    int ly[NS], lx[NS], kk[NS];
+    int ly[NS], lx[NS], kk[NS], ls[NS];
     bool own[NS], act[NS];
     uint32_t cc[NS];
     LapW tw[NS];
@@ -345,7 +373,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     float pa[NS][3], pb[NS][3], pka[NS][3], pkb[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl]);
+        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl], ls[sl]);
         if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
         kk[sl] = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
         const int q = min(max(kk[sl], 0), S.n - 1);
@@ -429,7 +457,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         if (!slot_in_stage<G>(sl, 2)) continue;
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
-        if (!own[sl]) continue;
+        if (ls[sl] < 2) continue;
         lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             float f3[3], yc[3], yh[3];
@@ -450,7 +478,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     __syncthreads();
 #pragma unroll
     for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<G>(sl, 2) && own[sl])
+        if (slot_in_stage<G>(sl, 2) && ls[sl] >= 2)
 #pragma unroll
             for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
@@ -461,7 +489,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
         if (!slot_in_stage<G>(sl, 3)) continue;
         float lt[3], zc[3], o[3];
         Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
-        if (!own[sl]) continue;
+        if (ls[sl] < 3) continue;
         lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             jtz(Jh[sl], zc, o);
@@ -476,7 +504,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     __syncthreads();
 #pragma unroll
     for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<G>(sl, 3) && own[sl])
+        if (slot_in_stage<G>(sl, 3) && ls[sl] >= 3)
 #pragma unroll
             for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();

@@ -798,6 +798,9 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     *out = nullptr;
     if (p->nrows < 4 || p->ncols < 4 || p->nt < 1 || !(p->dt > 0) || !std::isfinite(p->dt)) return FKADJ_ERR_ARG;
     if (p->precision != FKADJ_F64 && p->precision != FKADJ_F32) return FKADJ_ERR_ARG;
+    // kernels index a state with 32-bit offsets (bytes for the FP32 buffer loads)
+    if ((int64_t)p->nrows * ((p->ncols + 63) / 64 * 64) * 3 * (p->precision == FKADJ_F32 ? 4 : 8) >= (int64_t)1 << 31)
+        return FKADJ_ERR_ARG;
     if (p->fwd_tile < 0 || p->fwd_tile >= NUM_TILE_CFG || p->adj_tile < 0 || p->adj_tile >= NUM_TILE_CFG)
         return FKADJ_ERR_ARG;
     if (p->pulse_mode == FKADJ_PULSE_KNOTS) {

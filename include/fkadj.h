@@ -83,7 +83,7 @@ int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, con
                                     dJ/dA_k (trapezoid quadrature of alpha*E - z.gh per step) */
 
 typedef struct fkadj_problem {
-    int nrows, ncols;                 /* grid; any size >= 4 in each direction */
+    int nrows, ncols;                 /* grid; >= 4 each way, one state (3 planes) under 2 GiB */
     const int32_t *active_index;      /* nrows*ncols */
     const double *gh_num;             /* n_num values of ghost_lr (-1, 0, 1); NULL: derived from the mask */
     fkadj_model model;

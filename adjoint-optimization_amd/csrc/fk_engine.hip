@@ -210,14 +210,13 @@ void fill_models(fkadj_instance *I)
     b.ghc = (float)a.ghc;
 }
 
-// Built-in tile table, from measurements on MI355X at 512^2 with 1 and 16 instances:
-//  * adjoint: 64x16 cells per 512-thread workgroup (two tile cells per thread) beat 64x16/256,
-//    64x32/512 and 64x32/1024 at both sizes (occupancy matters more than the x1.5 halo work);
-//  * forward: 64x16/512 for a single instance, 64x32/512 once a launch covers about 2M cells
-//    (the smaller x1.15 halo recomputation pays when the GPU is full).  The forward tile does not
-//    change any result, so a group picks it from its total size.
+// Built-in tile table, from measurements on MI355X (grids 256^2..1024^2 at about 4M cells per
+// launch, and single 512^2 instances): 64x16 cells per 512-thread workgroup (two tile cells per
+// thread) is fastest for both kernels; 64x32/512 is within 2% for the forward step and 35% slower
+// for the adjoint step (register pressure), 64x16/256 and 64x32/1024 are slower for both.  The
+// forward tile does not change any result, so a group may pick it from its total size.
 int default_adj_cfg(int, int) { return TILE_16_512; }
-int default_fwd_cfg(int64_t group_cells) { return group_cells >= (int64_t)2 << 20 ? TILE_32_512 : TILE_16_512; }
+int default_fwd_cfg(int64_t) { return TILE_16_512; }
 
 // XCD-aware order: the GPU deals workgroups round-robin over its 8 XCDs, each with its own L2.
 // Block b runs on XCD b % 8; giving XCD x a contiguous run of tiles keeps neighbouring tiles

@@ -6,12 +6,16 @@ changes a result; the adjoint tile regroups the per-tile z.gh partial sums (roun
 so it is fixed per grid size.  Time blocking is one time step per launch: the kernels are bound by
 vector-ALU issue, not by HBM, so deeper temporal blocking would only add halo recomputation.
 
-    python -m fkadj.autotune --grid 512 --batch 16            # sweep, print a table
-    python -m fkadj.autotune --grid 512 --batch 16 --write    # store the best pair in tuned_tiles.json
+    python -m fkadj.autotune --grid 512 --batch 16            # sweep one size, print a table
+    python -m fkadj.autotune --sizes 256 384 512 768 1024 --write
+                                    # sweep each grid size at about 4M cells per launch and store
+                                    # the best pair per size in tuned_tiles.json
     python -m fkadj.autotune --grid 512 --batch 16 --once 3 4 # one evaluation (profiler target)
 
-The synthetic batch uses the benchmark's geometry (80 disc holes, h = 0.025 cm, dt = 0.02 ms,
-200 amplitudes) with a cheap initial state (half-plane excitation) and T = 20 ms.
+The synthetic batch uses the benchmark's geometry (80 disc holes per 512^2, h = 0.025 cm,
+dt = 0.02 ms, 200 amplitudes) with a cheap initial state (half-plane excitation) and T = 20 ms.
+Instances pick their tiles by grid size with lookup(N); the forward tile changes no result and
+the adjoint tile only the grouping of the FP64 partial sums.
 """
 from __future__ import annotations
 
@@ -31,8 +35,9 @@ FWD_CFGS = (1, 2, 3, 4, 5)
 ADJ_CFGS = (1, 3, 4, 5)
 
 
-def synthetic_batch(N: int, B: int, T: float, fwd_tile: int, adj_tile: int, holes: int = 80):
+def synthetic_batch(N: int, B: int, T: float, fwd_tile: int, adj_tile: int, holes: int | None = None):
     insts = []
+    holes = holes if holes is not None else max(1, round(80 * N * N / 512 ** 2))
     for s in range(B):
         m = W.holes_mask(N, holes, s)
         inst = W.make_instance(m, T, 200, precision=L.F32, ckpt=50, fwd_tile=fwd_tile, adj_tile=adj_tile)
@@ -64,29 +69,49 @@ def time_pair(N: int, B: int, T: float, ft: int, at: int, reps: int = 1) -> dict
             i.close()
 
 
-def load_table() -> dict:
-    if os.path.exists(TABLE):
-        with open(TABLE) as f:
+def load_table(path: str = TABLE) -> dict:
+    if os.path.exists(path):
+        with open(path) as f:
             return json.load(f)
     return {}
 
 
-def lookup(N: int, B: int) -> tuple[int, int]:
-    """Tuned (fwd_tile, adj_tile) for an N^2 grid in a batch of B, or (0, 0) for the built-in table."""
+def lookup(N: int) -> tuple[int, int]:
+    """Tuned (fwd_tile, adj_tile) for an N^2 grid: the entry of the nearest tuned size, or (0, 0)
+    (the library's built-in choice) when nothing is tuned."""
     t = load_table()
-    key = f"{N}x{B}"
-    if key in t:
-        return int(t[key]["fwd_tile"]), int(t[key]["adj_tile"])
-    return 0, 0
+    if not t:
+        return 0, 0
+    key = min(t, key=lambda k: abs(int(k) - N))
+    return int(t[key]["fwd_tile"]), int(t[key]["adj_tile"])
+
+
+def sweep(N: int, B: int, T: float) -> dict:
+    """Forward tile first (adjoint at the built-in choice), then the adjoint tile."""
+    res_f = {ft: time_pair(N, B, T, ft, 0) for ft in FWD_CFGS}
+    for ft, r in res_f.items():
+        print(f"{N}^2 x{B} fwd_tile={ft}: forward {r['fwd_us']:.1f} us/launch", flush=True)
+    best_f = min(res_f, key=lambda c: res_f[c]["fwd_us"])
+    res_a = {at: time_pair(N, B, T, best_f, at) for at in ADJ_CFGS}
+    for at, r in res_a.items():
+        print(f"{N}^2 x{B} adj_tile={at}: adjoint {r['adj_us']:.1f} us/launch, eval {1e3 * r['eval_s']:.1f} ms",
+              flush=True)
+    best_a = min(res_a, key=lambda c: res_a[c]["adj_us"])
+    print(f"{N}^2 x{B} best: fwd_tile={best_f} adj_tile={best_a}", flush=True)
+    return {"fwd_tile": best_f, "adj_tile": best_a, "batch": B, "fwd_us": res_f[best_f]["fwd_us"],
+            "adj_us": res_a[best_a]["adj_us"]}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--batch", type=int, default=16)
+    ap.add_argument("--sizes", type=int, nargs="+", help="sweep these grid sizes at --cells cells per launch")
+    ap.add_argument("--cells", type=int, default=1 << 22)
     ap.add_argument("--T", type=float, default=20.0)
     ap.add_argument("--once", type=int, nargs=2, metavar=("FWD", "ADJ"))
     ap.add_argument("--write", action="store_true")
+    ap.add_argument("--table", default=TABLE, help="table file read and written by --write")
     args = ap.parse_args()
     if args.once:
         insts, A = synthetic_batch(args.grid, args.batch, args.T, *args.once)
@@ -94,21 +119,13 @@ def main():
         batch_grad(insts, A)
         print(f"one evaluation: {time.perf_counter() - t0:.3f} s", flush=True)
         return
-    # forward tile first (adjoint at the built-in choice), then the adjoint tile
-    res_f = {ft: time_pair(args.grid, args.batch, args.T, ft, 0) for ft in FWD_CFGS}
-    for ft, r in res_f.items():
-        print(f"fwd_tile={ft}: forward {r['fwd_us']:.1f} us/launch", flush=True)
-    best_f = min(res_f, key=lambda c: res_f[c]["fwd_us"])
-    res_a = {at: time_pair(args.grid, args.batch, args.T, best_f, at) for at in ADJ_CFGS}
-    for at, r in res_a.items():
-        print(f"adj_tile={at}: adjoint {r['adj_us']:.1f} us/launch, eval {1e3 * r['eval_s']:.1f} ms", flush=True)
-    best_a = min(res_a, key=lambda c: res_a[c]["adj_us"])
-    print(f"best: fwd_tile={best_f} adj_tile={best_a}", flush=True)
+    sizes = args.sizes or [args.grid]
+    t = load_table(args.table)
+    for N in sizes:
+        B = args.batch if not args.sizes else max(1, round(args.cells / (N * N)))
+        t[str(N)] = sweep(N, B, args.T)
     if args.write:
-        t = load_table()
-        t[f"{args.grid}x{args.batch}"] = {"fwd_tile": best_f, "adj_tile": best_a,
-                                          "fwd_us": res_f[best_f]["fwd_us"], "adj_us": res_a[best_a]["adj_us"]}
-        with open(TABLE, "w") as f:
+        with open(args.table, "w") as f:
             json.dump(t, f, indent=1, sort_keys=True)
 
 

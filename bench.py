@@ -176,7 +176,9 @@ def run_c5(args):
     ps = W.phase_singularities(y0, m)
     log(f"[c5] four-spiral state ready in {time.perf_counter() - t0:.1f} s ({ps} phase singularities)")
     A = W.amplitudes(K_amp, seed)
-    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt)
+    from fkadj import autotune
+    ft, at = autotune.lookup(N)
+    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=ft, adj_tile=at)
     inst.set_state(y0)
     for _ in range(args.warmup):
         inst.grad(A)
@@ -212,7 +214,7 @@ def run_c5(args):
         Ts, Ks = 100.0, 100
         gs = {}
         for prec in (fkadj.F32, fkadj.F64):
-            ish = W.make_instance(m, Ts, Ks, precision=prec, ckpt=ckpt)
+            ish = W.make_instance(m, Ts, Ks, precision=prec, ckpt=ckpt, fwd_tile=ft, adj_tile=at)
             ish.set_state(y0)
             gs[prec] = ish.grad(A[:Ks])[0]
             ish.close()
@@ -264,9 +266,11 @@ def run_c3(args):
     masks, states = prepare(sizes, seeds)
     log(f"[c3] {B} states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
+    from fkadj import autotune
     insts = []
     for m, y in zip(masks, states):
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50)
+        ft, at = autotune.lookup(m.shape[0])       # tiles tuned per grid size
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at)
         inst.set_state(y)
         insts.append(inst)
     for w in range(args.warmup):
@@ -322,12 +326,14 @@ def run_c4(args):
     t0 = time.perf_counter()
     masks, states = prepare(sizes, seeds)
     log(f"[c4] {P} instance states ready in {time.perf_counter() - t0:.1f} s")
+    from fkadj import autotune
     insts = []
     for m, y, T in zip(masks, states, Ts):
+        ft, at = autotune.lookup(m.shape[0])
         inst = fkadj.Instance(fkadj.geometry.active_index_from_mask(m), m.shape[0], m.shape[1], dt=W.DT_MS,
                               nt=int(round(T / W.DT_MS)), model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE,
                               n_amp=T, alpha=2 * W.LAMBDA, gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()),
-                              precision=fkadj.F32, ckpt=50, lazy_workspace=True)
+                              precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at, lazy_workspace=True)
         inst.set_state(y)
         insts.append(inst)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
@@ -421,7 +427,7 @@ def main():
     log(f"[bench] states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
     from fkadj import autotune
-    fwd_tile, adj_tile = autotune.lookup(N, B)
+    fwd_tile, adj_tile = autotune.lookup(N)
     insts = []
     for m, y in zip(masks, states):
         inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)

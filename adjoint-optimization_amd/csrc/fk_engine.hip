@@ -122,6 +122,7 @@ struct fkadj_instance {
     bool state_set = false;
     bool lazy_ws = false;    // the default evaluation context is allocated on first use
     int busy = 0;            // server requests in flight on this instance
+    int plans = 0;           // batch plans holding this instance
 
     uint16_t *d_code = nullptr;
     int32_t *d_cell_of = nullptr;
@@ -924,7 +925,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
 int fkadj_destroy(fkadj_instance *I)
 {
     if (!I) return FKADJ_ERR_ARG;
-    if (I->busy) return FKADJ_ERR_BUSY;
+    if (I->busy || I->plans) return FKADJ_ERR_BUSY;
     if (I->stream) (void)hipStreamSynchronize(I->stream);
     dfree(I->d_code); dfree(I->d_cell_of); dfree(I->d_rowptr); dfree(I->d_colind); dfree(I->d_val); dfree(I->d_gamma);
     ctx_free(I->ctx);
@@ -1157,6 +1158,123 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
     (void)hipStreamSynchronize(s);
     group_free(g);
     return rc;
+}
+
+}  // extern "C"
+
+// ============================================================================ batch plans
+// A captured HIP graph of one whole evaluation of a fixed group.  The graph holds kernel and
+// device-to-device copy nodes only (no memset nodes); it reads the field tables and the resident
+// states at replay time, so a replay after new tables (or a new state) computes the new gradient.
+struct fkadj_plan {
+    std::vector<fkadj_instance *> insts;
+    FusedGroupDev *g = nullptr;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    hipStream_t stream = nullptr;
+};
+
+namespace {
+
+int batch_validate(int count, fkadj_instance *const *insts, const double *const *E)
+{
+    if (count < 1 || count > MAX_SLOTS || !insts) return FKADJ_ERR_ARG;
+    for (int k = 0; k < count; k++) {
+        fkadj_instance *I = insts[k];
+        if (!I || (E && !E[k])) return FKADJ_ERR_ARG;
+        if (I->prec != FKADJ_F32 || I->nt != insts[0]->nt || I->K != insts[0]->K) return FKADJ_ERR_ARG;
+        for (int q = 0; q < k; q++)
+            if (insts[q] == I) return FKADJ_ERR_ARG;
+        if (!I->state_set) return FKADJ_ERR_STATE;
+        if (I->busy) return FKADJ_ERR_BUSY;
+        if (E && !all_finite(E[k], I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    }
+    return FKADJ_OK;
+}
+
+void plan_free(fkadj_plan *P)
+{
+    if (!P) return;
+    if (P->stream) (void)hipStreamSynchronize(P->stream);
+    if (P->exec) (void)hipGraphExecDestroy(P->exec);
+    if (P->graph) (void)hipGraphDestroy(P->graph);
+    group_free(P->g);
+    for (auto *I : P->insts) I->plans--;
+    delete P;
+}
+
+}  // namespace
+
+extern "C" {
+
+int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out)
+{
+    if (!out) return FKADJ_ERR_ARG;
+    *out = nullptr;
+    CHK(batch_validate(count, insts, nullptr));
+    auto *P = new (std::nothrow) fkadj_plan();
+    if (!P) return FKADJ_ERR_NOMEM;
+    P->stream = insts[0]->stream;
+    std::vector<EvalCtx *> members;
+    for (int k = 0; k < count; k++) {
+        P->insts.push_back(insts[k]);
+        insts[k]->plans++;
+        int rc = ensure_default_ctx(insts[k]);
+        if (rc) { plan_free(P); return rc; }
+        members.push_back(insts[k]->ctx);
+    }
+    int rc = group_build(members, P->stream, &P->g);
+    if (rc) { plan_free(P); return rc; }
+    Sampler off(false);
+    if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+        plan_free(P);
+        return FKADJ_ERR_DEVICE;
+    }
+    rc = run_fused(P->g, P->stream, off);
+    hipGraph_t graph = nullptr;
+    hipError_t e = hipStreamEndCapture(P->stream, &graph);
+    P->graph = graph;
+    if (rc == FKADJ_OK && e != hipSuccess) rc = FKADJ_ERR_DEVICE;
+    if (rc == FKADJ_OK && hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0) != hipSuccess)
+        rc = FKADJ_ERR_DEVICE;
+    if (rc) {
+        (void)hipGetLastError();
+        plan_free(P);
+        return rc;
+    }
+    *out = P;
+    return FKADJ_OK;
+}
+
+int fkadj_plan_grad(fkadj_plan *P, const double *const *E, double *const *grad, double *Lcost, double *L0,
+                    double *LE)
+{
+    if (!P || !E) return FKADJ_ERR_ARG;
+    const int count = (int)P->insts.size();
+    CHK(batch_validate(count, P->insts.data(), E));
+    hipStream_t s = P->stream;
+    for (int k = 0; k < count; k++) CHK(build_tables(P->insts[k], P->insts[k]->ctx, E[k], s));
+    Sampler smp(false);
+    hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
+    HIPCHK(hipEventRecord(e0, s));
+    HIPCHK(hipGraphLaunch(P->exec, s));
+    HIPCHK(hipEventRecord(e1, s));
+    int64_t cells = 0;
+    for (int k = 0; k < count; k++) {
+        fkadj_instance *I = P->insts[k];
+        CHK(fetch_and_assemble(I->ctx, s, E[k], grad ? grad[k] : nullptr, Lcost ? Lcost + k : nullptr,
+                               L0 ? L0 + k : nullptr, LE ? LE + k : nullptr));
+        cells += (int64_t)I->rows * I->pitch * I->nt;
+    }
+    record_timing(smp, e0, e1, cells);
+    return FKADJ_OK;
+}
+
+int fkadj_plan_destroy(fkadj_plan *P)
+{
+    if (!P) return FKADJ_ERR_ARG;
+    plan_free(P);
+    return FKADJ_OK;
 }
 
 }  // extern "C"

@@ -87,6 +87,9 @@ def lib():
     L.fkadj_stimulate.argtypes = [vp, C.POINTER(C.c_uint8), C.c_double]
     L.fkadj_batch_grad.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
     L.fkadj_batch_forward.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.c_int]
+    L.fkadj_plan_create.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(vp)]
+    L.fkadj_plan_grad.argtypes = [vp, C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
+    L.fkadj_plan_destroy.argtypes = [vp]
     L.fkadj_server_create.argtypes = [C.c_int64, C.c_int, C.POINTER(vp)]
     L.fkadj_server_destroy.argtypes = [vp]
     L.fkadj_server_submit.argtypes = [vp, vp, dp, C.POINTER(C.c_int64)]

@@ -95,8 +95,10 @@ class Instance:
         self.n_controls = L.lib().fkadj_n_controls(h)
 
     def close(self):
+        """Frees the instance; raises FkadjError (ERR_BUSY) while a server request or a batch plan
+        still uses it (the instance then stays valid)."""
         if getattr(self, "h", None):
-            L.lib().fkadj_destroy(self.h)
+            check(L.lib().fkadj_destroy(self.h), "fkadj_destroy")
             self.h = None
 
     def __del__(self):
@@ -160,6 +162,45 @@ def batch_grad(instances, Es):
     Lc, L0, LE = np.zeros(n), np.zeros(n), np.zeros(n)
     check(L.lib().fkadj_batch_grad(n, hs, ep, gp, L._d(Lc), L._d(L0), L._d(LE)), "fkadj_batch_grad")
     return [(grads[k], Lc[k], L0[k], LE[k]) for k in range(n)]
+
+
+class Plan:
+    """A fixed batch of FP32 instances with one captured HIP graph of a whole evaluation
+    (fkadj_plan_*); ``grad(Es)`` replays it for new fields and returns what ``batch_grad`` returns."""
+
+    def __init__(self, instances):
+        self.instances = list(instances)
+        n = len(self.instances)
+        hs = (C.c_void_p * n)(*[inst.h for inst in self.instances])
+        h = C.c_void_p()
+        check(L.lib().fkadj_plan_create(n, hs, C.byref(h)), "fkadj_plan_create")
+        self.h = h.value
+
+    def grad(self, Es):
+        n = len(self.instances)
+        Es = [L._f64(np.ravel(E)) for E in Es]
+        if len(Es) != n:
+            raise ValueError(f"need {n} fields")
+        grads = [np.zeros(inst.n_controls) for inst in self.instances]
+        ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
+        gp = (C.POINTER(C.c_double) * n)(*[L._d(g) for g in grads])
+        Lc, L0, LE = np.zeros(n), np.zeros(n), np.zeros(n)
+        check(L.lib().fkadj_plan_grad(self.h, ep, gp, L._d(Lc), L._d(L0), L._d(LE)), "fkadj_plan_grad")
+        return [(grads[k], Lc[k], L0[k], LE[k]) for k in range(n)]
+
+    def close(self):
+        if getattr(self, "h", None):
+            L.lib().fkadj_plan_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        self.close()
 
 
 def batch_forward(instances, Es, nsteps):

@@ -105,9 +105,10 @@ typedef struct fkadj_problem {
                                          0 -> built-in choice by the cells of the launch (the
                                          forward tile never changes a result) */
     int adj_tile;                     /* FP32 adjoint tile, same ids; 0 -> built-in table */
-    int lazy_workspace;               /* 1: checkpoint/segment/adjoint buffers are allocated on first use
-                                         and released after each request a server completes, so a
-                                         server's memory budget covers only admitted requests */
+    int lazy_workspace;               /* 1: the evaluation buffers of fkadj_grad / fkadj_forward /
+                                         batch calls are allocated on first use (a server gives each
+                                         request its own, so an instance used only through a server
+                                         holds just its resident data) */
 } fkadj_problem;
 
 typedef struct fkadj_instance fkadj_instance;
@@ -152,6 +153,19 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
  * each final state replaces the instance's resident state (used to prepare initial states). */
 int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *const *E, int nsteps);
 
+/* A batch plan: the launch lists of fkadj_batch_grad for a fixed set of instances and one HIP
+ * graph holding a whole evaluation (forward sweep, objective, recompute and adjoint segments,
+ * sums), captured once and replayed per call.  Only the field tables move per call (host to
+ * device, before the replay).  The plan uses the instances' own evaluation buffers: while it
+ * exists the instances cannot be destroyed (FKADJ_ERR_BUSY), and calls on them must not overlap
+ * a plan call.  Results equal fkadj_batch_grad (and fkadj_grad) bit for bit.  No reference
+ * counterpart. */
+typedef struct fkadj_plan fkadj_plan;
+int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out);
+int fkadj_plan_grad(fkadj_plan *plan, const double *const *E, double *const *grad, double *Lcost, double *L0,
+                    double *LE);
+int fkadj_plan_destroy(fkadj_plan *plan);
+
 /* ---------------------------------------------------------------- timing */
 typedef struct fkadj_timing {
     double eval_ms;          /* last gradient evaluation, device time between HIP events */
@@ -166,20 +180,22 @@ int fkadj_set_profiling(int on);
 int fkadj_last_timing(fkadj_timing *t);
 
 /* ---------------------------------------------------------------- serving */
-/* Continuous-batching gradient server on one GPU (one host thread drives it).  Requests are
- * admitted when a slot is free and their device footprint (fkadj_footprint_bytes) fits the
- * remaining budget; admitted requests advance together, one forward-kind and one adjoint launch
- * per round over all slots, each slot at its own step and phase; new requests join and finished
- * ones leave at round boundaries.  Each result equals the same request evaluated alone
- * (fkadj_grad on the instance) bit for bit.  Instances created with lazy_workspace hold their
- * checkpoint buffers only while admitted.  No reference counterpart (the reference evaluates one
- * gradient per synchronous MEX call). */
+/* Continuous-batching gradient server on one GPU (one host thread drives it).  A request is
+ * admitted when a slot is free and its evaluation buffers (checkpoints, segment states, slopes,
+ * adjoint states, tables: the per-evaluation part of fkadj_footprint_bytes) fit the remaining
+ * budget; the buffers are allocated at admission and freed when the request finishes.  Admitted
+ * requests advance together, one forward-kind and one adjoint launch per round over all slots,
+ * each slot at its own step and phase; new requests join and finished ones leave at round
+ * boundaries.  Each result equals the same request evaluated alone (fkadj_grad on the instance)
+ * bit for bit.  No reference counterpart (the reference evaluates one gradient per synchronous
+ * MEX call). */
 typedef struct fkadj_server fkadj_server;
 
 int fkadj_server_create(int64_t budget_bytes, int max_slots /* <= 64 */, fkadj_server **out);
 int fkadj_server_destroy(fkadj_server *s);
-/* Queue a gradient request on an FP32 instance whose state is set; E is copied.  Requests on one
- * instance run one at a time, in order; the instance must not be used by other calls meanwhile. */
+/* Queue a gradient request on an FP32 instance whose state is set; E is copied.  Several requests
+ * on one instance run side by side (each has its own buffers); while any is queued or running,
+ * fkadj_set_state, fkadj_stimulate, fkadj_forward and fkadj_destroy on it return FKADJ_ERR_BUSY. */
 int fkadj_server_submit(fkadj_server *s, fkadj_instance *inst, const double *E, int64_t *request_id);
 /* A queued request is dropped; a running one leaves at the next round boundary (others unaffected). */
 int fkadj_server_cancel(fkadj_server *s, int64_t request_id);

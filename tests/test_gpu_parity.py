@@ -12,6 +12,7 @@ import numpy as np
 import pytest
 
 import fkadj
+from fkadj import _lib as L
 from fkadj import geometry as G
 
 from cases import crop_case, disc_case, golden, oracle_grad_knots, oracle_grad_piecewise, relerr
@@ -132,6 +133,46 @@ def test_batch_equals_alone():
     mixed = fkadj.batch_grad(insts, Es)
     for a, b in zip(alone, mixed):
         assert np.array_equal(a[0], b[0]) and a[1] == b[1] and a[2] == b[2]
+    for inst in insts:
+        inst.close()
+
+
+def test_graph_plan_replays_equal_eager():
+    """A captured batch plan replayed several times (new fields, a new resident state, eager calls
+    and synchronisations in between) gives the eager batched results bit for bit every time."""
+    _print_case("graph_plan", sizes="(64,64),(96,128),(40,192)")
+    insts, E1, E2 = [], [], []
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K = 60, 0.02, 6
+    cases = []
+    for k, (H, W) in enumerate([(64, 64), (96, 128), (40, 192)]):
+        case = disc_case(H, W, 6 + 2 * k, 10 + k)
+        inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                              alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=20)
+        inst.set_state(case.y0)
+        insts.append(inst)
+        cases.append(case)
+        E1.append(np.random.default_rng(k).uniform(0, 10, K))
+        E2.append(np.random.default_rng(50 + k).uniform(0, 10, K))
+
+    def same(a, b):
+        return all(np.array_equal(x[0], y[0]) and x[1] == y[1] and x[2] == y[2] for x, y in zip(a, b))
+
+    eager1, eager2 = fkadj.batch_grad(insts, E1), fkadj.batch_grad(insts, E2)
+    with fkadj.Plan(insts) as plan:
+        assert same(plan.grad(E1), eager1)
+        insts[1].grad(E2[1])                                   # eager work between replays
+        assert same(plan.grad(E2), eager2)
+        assert same(plan.grad(E1), eager1)
+        with pytest.raises(fkadj.FkadjError) as e:             # held by the plan
+            insts[0].close()
+        assert e.value.code == L.ERR_BUSY
+        y_new = cases[0].y0.copy()
+        y_new[: cases[0].n_act] *= 0.5                        # a new resident state is read at replay
+        insts[0].set_state(y_new)
+        eager3 = fkadj.batch_grad(insts, E1)
+        assert same(plan.grad(E1), eager3)
+        assert not same(eager3, eager1)
     for inst in insts:
         inst.close()
 

@@ -24,6 +24,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
 
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per cell of one launch of the fused kernels (64x16/512 tiles), measured with rocprofv3
+# --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2 batch and scaled by the factors a 1 GiB copy kernel
+# gives those counters (x2.0 and x1.0); forward is the mean of forward-sweep and recompute launches
+HBM_BYTES_PER_CELL = {"adj": 75.5, "fwd": 32.2}
 
 
 def log(*a):
@@ -101,13 +105,13 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
     fwd_bytes = 7 * 4 * cells
     adj_bytes = 13 * 4 * cells
     if adj_ms * na >= fwd_ms * nf:
-        dom, dbytes, dms = "k_fused_adj (adjoint step)", adj_bytes, adj_ms
+        dom, dbytes, dms, dkey = "k_fused_adj (adjoint step)", adj_bytes, adj_ms, "adj"
     else:
-        dom, dbytes, dms = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms
+        dom, dbytes, dms, dkey = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms, "fwd"
     achieved = dbytes / (dms * 1e-3) / 1e9
     eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": None, "kernel": dom,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": HBM_BYTES_PER_CELL[dkey] * cells, "kernel": dom,
             "kernel_ms": dms, "bytes_per_launch": dbytes,
             "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
             "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},

@@ -98,3 +98,42 @@ def test_disc_holes_have_no_isthmus_and_closed_form_ghost():
     assert set(np.unique(gh)) <= {-1.0, 0.0, 1.0}
     L_ = G.lap_matrix(m)
     assert np.allclose(L_.sum(axis=1), 0.0)   # mirror Laplacian annihilates constants
+
+
+def test_plan_and_server_argument_errors_before_device_work():
+    lib = fkadj.lib()
+    import ctypes as C
+    h = C.c_void_p()
+    assert lib.fkadj_plan_create(0, None, C.byref(h)) == L.ERR_ARG
+    assert lib.fkadj_plan_create(65, None, C.byref(h)) == L.ERR_ARG
+    assert lib.fkadj_plan_grad(None, None, None, None, None, None) == L.ERR_ARG
+    assert lib.fkadj_plan_destroy(None) == L.ERR_ARG
+    assert lib.fkadj_server_create(0, 4, C.byref(h)) == L.ERR_ARG
+    assert lib.fkadj_server_create(1 << 30, 65, C.byref(h)) == L.ERR_ARG
+    assert lib.fkadj_server_create(1 << 30, 0, C.byref(h)) == L.ERR_ARG
+    rid = C.c_int64()
+    assert lib.fkadj_server_submit(None, None, None, C.byref(rid)) == L.ERR_ARG
+    assert lib.fkadj_server_run(None, 0) == L.ERR_ARG
+    assert lib.fkadj_server_pending(None) == L.ERR_ARG
+
+
+def test_grid_too_large_for_32_bit_offsets():
+    p = L.Problem()
+    p.nrows, p.ncols, p.nt, p.dt, p.precision = 15000, 15000, 10, 0.1, L.F32
+    import ctypes as C
+    ai = np.ones(4, np.int32)   # never read: the size check comes first
+    p.active_index = ai.ctypes.data_as(C.POINTER(C.c_int32))
+    h = C.c_void_p()
+    assert fkadj.lib().fkadj_create(C.byref(p), C.byref(h)) == L.ERR_ARG
+
+
+def test_tile_table_lookup(tmp_path, monkeypatch):
+    from fkadj import autotune
+    t = tmp_path / "t.json"
+    t.write_text('{"256": {"fwd_tile": 1, "adj_tile": 4}, "1024": {"fwd_tile": 3, "adj_tile": 1}}')
+    monkeypatch.setattr(autotune, "load_table", lambda path=None: autotune.json.loads(t.read_text()))
+    assert autotune.lookup(256) == (1, 4)
+    assert autotune.lookup(300) == (1, 4)
+    assert autotune.lookup(2048) == (3, 1)
+    monkeypatch.setattr(autotune, "load_table", lambda path=None: {})
+    assert autotune.lookup(512) == (0, 0)

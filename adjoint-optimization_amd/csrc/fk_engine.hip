@@ -161,9 +161,8 @@ struct EvalCtx {
 struct FusedGroupDev {
     std::vector<EvalCtx *> members;
     FusedSlot *d_slots = nullptr;
-    // one tile list per kernel configuration (index = tile config id)
-    TileTask *d_fwd[NUM_TILE_CFG] = {}, *d_adj[NUM_TILE_CFG] = {};
-    int nfwd[NUM_TILE_CFG] = {}, nadj[NUM_TILE_CFG] = {};
+    // one launch map per kernel configuration (index = tile config id); slot k = member k
+    LaunchMap fwd[NUM_TILE_CFG], adj[NUM_TILE_CFG];
 };
 
 namespace {
@@ -218,28 +217,6 @@ void fill_models(fkadj_instance *I)
 // forward tile does not change any result, so a group may pick it from its total size.
 int default_adj_cfg(int, int) { return TILE_16_512; }
 int default_fwd_cfg(int64_t) { return TILE_16_512; }
-
-// XCD-aware order: the GPU deals workgroups round-robin over its 8 XCDs, each with its own L2.
-// Block b runs on XCD b % 8; giving XCD x a contiguous run of tiles keeps neighbouring tiles
-// (which share halo cache lines) in the same L2.  The map is a bijection for any count.
-std::vector<TileTask> xcd_order(const std::vector<TileTask> &in)
-{
-    const size_t X = 8, N = in.size();
-    std::vector<size_t> start(X + 1, 0);
-    for (size_t x = 0; x < X; x++) start[x + 1] = start[x] + (N > x ? (N - x + X - 1) / X : 0);
-    std::vector<TileTask> out(N);
-    for (size_t b = 0; b < N; b++) out[b] = in[start[b % X] + b / X];
-    return out;
-}
-
-std::vector<TileTask> make_tiles(int slot, int rows, int pitch, int ty)
-{
-    std::vector<TileTask> v;
-    int idx = 0;
-    for (int y0 = 0; y0 < rows; y0 += ty)
-        for (int x0 = 0; x0 < pitch; x0 += FUSED_TX) v.push_back(TileTask{slot, x0, y0, idx++});
-    return v;
-}
 
 int validate_and_codes(const fkadj_problem *p, int pitch, std::vector<uint16_t> &code, std::vector<int32_t> &cell_of,
                        int &n_num)
@@ -585,12 +562,10 @@ void group_free(FusedGroupDev *g)
 {
     if (!g) return;
     if (g->d_slots) (void)hipFree(g->d_slots);
-    for (int c = 0; c < NUM_TILE_CFG; c++) {
-        if (g->d_fwd[c]) (void)hipFree(g->d_fwd[c]);
-        if (g->d_adj[c]) (void)hipFree(g->d_adj[c]);
-    }
     delete g;
 }
+
+int fwd_cfg_of(const fkadj_instance *I) { return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(0); }
 
 int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
 {
@@ -599,30 +574,19 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
     if (!g) return FKADJ_ERR_NOMEM;
     g->members = members;
     std::vector<FusedSlot> slots;
-    std::vector<TileTask> fl[NUM_TILE_CFG], al[NUM_TILE_CFG];
-    int64_t cells = 0;
-    for (auto *C : members) cells += (int64_t)C->I->rows * C->I->pitch;
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
+        launch_map_clear(g->fwd[c]);
+        launch_map_clear(g->adj[c]);
+    }
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k]->I;
         slots.push_back(members[k]->slot_h);
-        const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
-        auto ft = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(fc));
-        fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
-        auto at = make_tiles((int)k, I->rows, I->pitch, tile_rows_of(I->adj_cfg));
-        al[I->adj_cfg].insert(al[I->adj_cfg].end(), at.begin(), at.end());
+        const int fc = fwd_cfg_of(I);
+        launch_map_add(g->fwd[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
+        launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
     }
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
-    for (int c = 0; c < NUM_TILE_CFG && ok; c++) {
-        for (int side = 0; side < 2 && ok; side++) {
-            std::vector<TileTask> v = xcd_order(side == 0 ? fl[c] : al[c]);
-            if (v.empty()) continue;
-            TileTask *&d = side == 0 ? g->d_fwd[c] : g->d_adj[c];
-            (side == 0 ? g->nfwd[c] : g->nadj[c]) = (int)v.size();
-            ok = hipMalloc((void **)&d, sizeof(TileTask) * v.size()) == hipSuccess &&
-                 hipMemcpyAsync(d, v.data(), sizeof(TileTask) * v.size(), hipMemcpyHostToDevice, s) == hipSuccess;
-        }
-    }
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) { group_free(g); return FKADJ_ERR_NOMEM; }
     *out = g;
@@ -639,15 +603,13 @@ Cursors uniform_cursors(int count, int mode, int step, int seg_a)
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
-    for (int k = 1; k < NUM_TILE_CFG; k++)
-        if (g->nfwd[k]) fused_forward_launch(k, g->d_slots, g->d_fwd[k], g->nfwd[k], c, s);
+    for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
 }
 
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
-    for (int k = 1; k < NUM_TILE_CFG; k++)
-        if (g->nadj[k]) fused_adjoint_launch(k, g->d_slots, g->d_adj[k], g->nadj[k], c, s);
+    for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj[k], c, s);
 }
 
 int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
@@ -1318,11 +1280,9 @@ struct fkadj_server {
     int max_slots = 0;
     hipStream_t stream = nullptr;
     FusedSlot *d_slots = nullptr;
+    FusedSlot *h_slots = nullptr;          // pinned staging of the slot records
+    hipEvent_t slot_ev[MAX_SLOTS] = {};    // completion of each slot record's last upload
     std::vector<ServReq *> slot_req;
-    TileTask *d_fwd[NUM_TILE_CFG] = {}, *d_adj[NUM_TILE_CFG] = {};
-    int nfwd[NUM_TILE_CFG] = {}, nadj[NUM_TILE_CFG] = {};
-    size_t cap_fwd[NUM_TILE_CFG] = {}, cap_adj[NUM_TILE_CFG] = {};
-    bool dirty = false;
     std::deque<ServReq *> queue;
     std::vector<ServReq *> finishing;
     std::map<int64_t, ServReq *> reqs;
@@ -1333,45 +1293,15 @@ struct fkadj_server {
 
 namespace {
 
-// Task lists and slot table for the occupied slots (after the stream has drained).
-int server_rebuild(fkadj_server *S)
+// A slot's record is uploaded in stream order when a request takes the slot: kernels of the
+// previous occupant, enqueued earlier, still read the old record.  The staging copy is reused only
+// after its previous upload has completed.
+int server_upload_slot(fkadj_server *S, int k, const FusedSlot &rec)
 {
-    HIPCHK(hipStreamSynchronize(S->stream));
-    std::vector<FusedSlot> slots(S->max_slots);
-    std::vector<TileTask> fl[NUM_TILE_CFG], al[NUM_TILE_CFG];
-    int64_t cells = 0;
-    for (int k = 0; k < S->max_slots; k++)
-        if (S->slot_req[k]) cells += (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
-    for (int k = 0; k < S->max_slots; k++) {
-        ServReq *r = S->slot_req[k];
-        if (!r) continue;
-        fkadj_instance *I = r->I;
-        slots[k] = r->C->slot_h;
-        const int fc = I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(cells);
-        auto ft = make_tiles(k, I->rows, I->pitch, tile_rows_of(fc));
-        fl[fc].insert(fl[fc].end(), ft.begin(), ft.end());
-        auto at = make_tiles(k, I->rows, I->pitch, tile_rows_of(I->adj_cfg));
-        al[I->adj_cfg].insert(al[I->adj_cfg].end(), at.begin(), at.end());
-    }
-    HIPCHK(hipMemcpy(S->d_slots, slots.data(), sizeof(FusedSlot) * S->max_slots, hipMemcpyHostToDevice));
-    for (int c = 0; c < NUM_TILE_CFG; c++) {
-        for (int side = 0; side < 2; side++) {
-            std::vector<TileTask> v = xcd_order(side == 0 ? fl[c] : al[c]);
-            TileTask *&d = side == 0 ? S->d_fwd[c] : S->d_adj[c];
-            size_t &cap = side == 0 ? S->cap_fwd[c] : S->cap_adj[c];
-            (side == 0 ? S->nfwd[c] : S->nadj[c]) = (int)v.size();
-            if (v.empty()) continue;
-            if (v.size() > cap) {
-                if (d) (void)hipFree(d);
-                d = nullptr;
-                cap = 0;
-                if (hipMalloc((void **)&d, sizeof(TileTask) * v.size()) != hipSuccess) return FKADJ_ERR_NOMEM;
-                cap = v.size();
-            }
-            HIPCHK(hipMemcpy(d, v.data(), sizeof(TileTask) * v.size(), hipMemcpyHostToDevice));
-        }
-    }
-    S->dirty = false;
+    HIPCHK(hipEventSynchronize(S->slot_ev[k]));
+    S->h_slots[k] = rec;
+    HIPCHK(hipMemcpyAsync(S->d_slots + k, S->h_slots + k, sizeof(FusedSlot), hipMemcpyHostToDevice, S->stream));
+    HIPCHK(hipEventRecord(S->slot_ev[k], S->stream));
     return FKADJ_OK;
 }
 
@@ -1393,7 +1323,6 @@ int server_finish(fkadj_server *S, ServReq *r, bool cancelled)
     r->slot = -1;
     r->state = RQ_FINISHING;
     S->finishing.push_back(r);
-    S->dirty = true;
     return FKADJ_OK;
 }
 
@@ -1458,6 +1387,7 @@ int server_admit(fkadj_server *S)
         if (rc == FKADJ_OK &&
             hipMemcpyAsync(I->state(r->C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, S->stream) != hipSuccess)
             rc = FKADJ_ERR_DEVICE;
+        if (rc == FKADJ_OK) rc = server_upload_slot(S, free_slot, r->C->slot_h);
         S->queue.erase(S->queue.begin() + q);
         if (rc != FKADJ_OK) {
             r->rc = rc;
@@ -1477,34 +1407,40 @@ int server_admit(fkadj_server *S)
         r->phase = PH_FWD;
         r->step = 0;
         S->slot_req[free_slot] = r;
-        S->dirty = true;
     }
     return FKADJ_OK;
 }
 
-// One round: one forward-kind and one adjoint launch over all busy slots, then every slot's phase
-// machine advances by one step.
+// One round: one forward-kind launch over the slots in a forward or recompute phase and one
+// adjoint launch over the slots in an adjoint phase (per tile configuration; each launch lists only
+// its slots), then every slot's phase machine advances by one step.
 int server_round(fkadj_server *S)
 {
     Cursors fc, ac, rc_rhs;
+    LaunchMap fm[NUM_TILE_CFG], am[NUM_TILE_CFG], rm[NUM_TILE_CFG];
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
+        launch_map_clear(fm[c]);
+        launch_map_clear(am[c]);
+        launch_map_clear(rm[c]);
+    }
     for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
-    bool any_f = false, any_a = false;
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];
         if (!r) continue;
-        const int K = r->I->K;
-        if (r->phase == PH_FWD) { fc.c[k] = SlotCursor{MODE_FWD, r->step, 0, 0}; any_f = true; }
-        else if (r->phase == PH_REC) { fc.c[k] = SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0}; any_f = true; }
-        else { ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0}; any_a = true; }
+        const fkadj_instance *I = r->I;
+        const int K = I->K, f = fwd_cfg_of(I);
+        if (r->phase == PH_ADJ) {
+            ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
+            launch_map_add(am[I->adj_cfg], k, tiles_of(I->rows, I->pitch, I->adj_cfg));
+        } else {
+            fc.c[k] = r->phase == PH_FWD ? SlotCursor{MODE_FWD, r->step, 0, 0}
+                                         : SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0};
+            launch_map_add(fm[f], k, tiles_of(I->rows, I->pitch, f));
+        }
     }
-    if (any_f)
-        for (int c = 1; c < NUM_TILE_CFG; c++)
-            if (S->nfwd[c]) fused_forward_launch(c, S->d_slots, S->d_fwd[c], S->nfwd[c], fc, S->stream);
-    if (any_a)
-        for (int c = 1; c < NUM_TILE_CFG; c++)
-            if (S->nadj[c]) fused_adjoint_launch(c, S->d_slots, S->d_adj[c], S->nadj[c], ac, S->stream);
+    for (int c = 1; c < NUM_TILE_CFG; c++) fused_forward_launch(c, S->d_slots, fm[c], fc, S->stream);
+    for (int c = 1; c < NUM_TILE_CFG; c++) fused_adjoint_launch(c, S->d_slots, am[c], ac, S->stream);
     HIPCHK(hipGetLastError());
-    bool any_rhs = false;
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];
         if (!r) continue;
@@ -1514,7 +1450,7 @@ int server_round(fkadj_server *S)
             if (++r->step == nt) {
                 terminal<float>(r->C, S->stream);               // objective, z(T), z.gh at T
                 rc_rhs.c[k] = SlotCursor{MODE_RHS, nt - 1, 0, 0};
-                any_rhs = true;
+                launch_map_add(rm[fwd_cfg_of(I)], k, tiles_of(I->rows, I->pitch, fwd_cfg_of(I)));
                 r->phase = PH_REC;
                 r->seg = (nt + K - 1) / K - 1;
                 r->step = 0;
@@ -1530,9 +1466,7 @@ int server_round(fkadj_server *S)
             }
         }
     }
-    if (any_rhs)
-        for (int c = 1; c < NUM_TILE_CFG; c++)
-            if (S->nfwd[c]) fused_forward_launch(c, S->d_slots, S->d_fwd[c], S->nfwd[c], rc_rhs, S->stream);
+    for (int c = 1; c < NUM_TILE_CFG; c++) fused_forward_launch(c, S->d_slots, rm[c], rc_rhs, S->stream);
     HIPCHK(hipGetLastError());
     // cancellations take effect at the round boundary
     for (int k = 0; k < S->max_slots; k++) {
@@ -1555,9 +1489,13 @@ int fkadj_server_create(int64_t budget_bytes, int max_slots, fkadj_server **out)
     S->max_slots = max_slots;
     S->slot_req.assign(max_slots, nullptr);
     bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
-              hipMalloc((void **)&S->d_slots, sizeof(FusedSlot) * max_slots) == hipSuccess;
+              hipMalloc((void **)&S->d_slots, sizeof(FusedSlot) * max_slots) == hipSuccess &&
+              hipHostMalloc((void **)&S->h_slots, sizeof(FusedSlot) * max_slots, hipHostMallocDefault) == hipSuccess;
     for (int k = 0; k < kRoundsInFlight && ok; k++)
         ok = hipEventCreateWithFlags(&S->round_ev[k], hipEventDisableTiming) == hipSuccess;
+    for (int k = 0; k < max_slots && ok; k++)   // recorded once, so the first upload need not wait
+        ok = hipEventCreateWithFlags(&S->slot_ev[k], hipEventDisableTiming) == hipSuccess &&
+             hipEventRecord(S->slot_ev[k], S->stream) == hipSuccess;
     if (!ok) {
         fkadj_server_destroy(S);
         return FKADJ_ERR_DEVICE;
@@ -1580,13 +1518,12 @@ int fkadj_server_destroy(fkadj_server *S)
         if (r->done) (void)hipEventDestroy(r->done);
         delete r;
     }
-    for (int c = 0; c < NUM_TILE_CFG; c++) {
-        if (S->d_fwd[c]) (void)hipFree(S->d_fwd[c]);
-        if (S->d_adj[c]) (void)hipFree(S->d_adj[c]);
-    }
     if (S->d_slots) (void)hipFree(S->d_slots);
+    if (S->h_slots) (void)hipHostFree(S->h_slots);
     for (int k = 0; k < kRoundsInFlight; k++)
         if (S->round_ev[k]) (void)hipEventDestroy(S->round_ev[k]);
+    for (int k = 0; k < MAX_SLOTS; k++)
+        if (S->slot_ev[k]) (void)hipEventDestroy(S->slot_ev[k]);
     if (S->stream) (void)hipStreamDestroy(S->stream);
     delete S;
     return FKADJ_OK;
@@ -1642,7 +1579,6 @@ int fkadj_server_run(fkadj_server *S, int max_rounds)
             }
             break;
         }
-        if (S->dirty) CHK(server_rebuild(S));
         if (S->rounds >= kRoundsInFlight) HIPCHK(hipEventSynchronize(S->round_ev[S->rounds % kRoundsInFlight]));
         CHK(server_round(S));
         HIPCHK(hipEventRecord(S->round_ev[S->rounds % kRoundsInFlight], S->stream));

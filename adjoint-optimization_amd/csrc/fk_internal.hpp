@@ -88,9 +88,6 @@ struct FusedSlot {
     float dt;
 };
 
-struct TileTask {
-    int slot, x0, y0, tile;     // instance, tile origin (cells), tile index within the instance
-};
 
 // MODE_FWD: forward sweep (checkpoints + ring); MODE_REC: segment recompute, also stores the
 // stage-1 slopes for the adjoint's Hermite reconstruction; MODE_RHS: only f(y_nt) -> kT;
@@ -108,12 +105,36 @@ struct Cursors {
     SlotCursor c[MAX_SLOTS];
 };
 
-// Launchers (stream-ordered, no synchronisation); cfg is a tile configuration.  Tiles whose
-// slot cursor is MODE_IDLE (or not the kernel's kind) return at once.
-void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          const Cursors &cur, hipStream_t stream);
-void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          const Cursors &cur, hipStream_t stream);
+// The tiles of one launch, passed by value: the launch's slots in order and the running count of
+// their tiles (tile t of slot[i] is launch tile first[i] + t; an instance's tiles are numbered
+// row-major, 64 x TY cells each).  first[count] is the grid size.  Only slots with work in the
+// launch are listed, so no workgroup is spent on an idle instance.
+struct LaunchMap {
+    int count;
+    int first[MAX_SLOTS + 1];
+    int slot[MAX_SLOTS];
+};
+inline void launch_map_clear(LaunchMap &m)
+{
+    m.count = 0;
+    m.first[0] = 0;
+}
+inline void launch_map_add(LaunchMap &m, int slot, int tiles)
+{
+    m.slot[m.count] = slot;
+    m.first[m.count + 1] = m.first[m.count] + tiles;
+    m.count++;
+}
+inline int launch_map_tiles(const LaunchMap &m) { return m.first[m.count]; }
+// tiles of a rows x pitch instance in tile configuration cfg
+inline int tiles_of(int rows, int pitch, int cfg) { return (pitch / FUSED_TX) * ((rows + tile_rows_of(cfg) - 1) / tile_rows_of(cfg)); }
+
+// Launchers (stream-ordered, no synchronisation; nothing is launched for an empty map); cfg is a
+// tile configuration.  Tiles whose slot cursor is not of the kernel's kind return at once.
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
+                          hipStream_t stream);
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
+                          hipStream_t stream);
 
 // ---- common small kernels (fk_kernels_common.hip)
 // z (terminal adjoint, T planes) and L0 partials from y(T): CSR S over packed indices or s_diag*I.

@@ -157,6 +157,35 @@ __device__ inline void jtz(const Jac32 &J, const float *z, float *o)
     o[2] = J.a20 * z[0] + J.a22 * z[2];
 }
 
+// Launch tile of this workgroup.  Workgroups are dealt round-robin over the GPU's 8 XCDs (block b
+// runs on XCD b % 8, each XCD with its own L2); XCD x takes a contiguous run of launch tiles, so
+// tiles that share halo lines share an L2.  The map is a bijection for any grid size.
+struct TileRef {
+    int slot, x0, y0, tile;   // instance slot, tile origin (cells), tile index within the instance
+};
+
+template <int TY>
+__device__ inline TileRef launch_tile(const FusedSlot *__restrict__ slots, const LaunchMap &map)
+{
+    const int n = map.first[map.count];
+    const int b = blockIdx.x, x = b & 7;
+    int lin = b >> 3;
+    for (int q = 0; q < x; q++) lin += (n - q + 7) >> 3;
+    int lo = 0, hi = map.count - 1;   // last i with first[i] <= lin
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (map.first[mid] <= lin) lo = mid;
+        else hi = mid - 1;
+    }
+    TileRef t;
+    t.slot = map.slot[lo];
+    t.tile = lin - map.first[lo];
+    const int ntx = slots[t.slot].pitch / FUSED_TX;
+    t.x0 = (t.tile % ntx) * FUSED_TX;
+    t.y0 = (t.tile / ntx) * TY;
+    return t;
+}
+
 // Region (tile + 4-cell halo) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
 // first 64 cells of a row (coalesced), lanes 0..7 the last 8 of 8 rows.  Cells outside the grid
 // read as 0 with code 0 (inactive).  All loads are issued before the first LDS store (their
@@ -200,15 +229,14 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 
 // ------------------------------------------------------------------------------ forward
 template <int TY, int NT>
-__global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ slots,
-                                                  const TileTask *__restrict__ tasks, Cursors cur)
+__global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT;
     __shared__ float sY[3][RY][RX];
     __shared__ uint16_t sC[RY][RX];
 
-    const TileTask t = tasks[blockIdx.x];
+    const TileRef t = launch_tile<TY>(slots, map);
     const SlotCursor sc = cur.c[t.slot];
     if (sc.mode != MODE_FWD && sc.mode != MODE_REC && sc.mode != MODE_RHS) return;   // uniform per block
     const int mode = sc.mode, step = sc.step, seg_a = sc.seg_a;
@@ -329,8 +357,7 @@ __device__ inline T h4p(T x0, T x1, T m0, T m1)
 }
 
 template <int TY, int NT>
-__global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ slots,
-                                                  const TileTask *__restrict__ tasks, Cursors cur)
+__global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
@@ -338,7 +365,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
     __shared__ float sB[3][RY][RX];   // y(t_{i-1} + dt/3)
     __shared__ uint16_t sC[RY][RX];
 
-    const TileTask t = tasks[blockIdx.x];
+    const TileRef t = launch_tile<TY>(slots, map);
     const SlotCursor sc = cur.c[t.slot];
     if (sc.mode != MODE_ADJ) return;   // uniform per block
     const int i = sc.step, seg_a = sc.seg_a;
@@ -537,27 +564,31 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 // ------------------------------------------------------------------------------ launchers
 // Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
 // 4: 64x16 / 512, 5: 64x32 / 1024.
-void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          const Cursors &cur, hipStream_t stream)
+void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
+                          hipStream_t stream)
 {
+    const dim3 grid(launch_map_tiles(map));
+    if (grid.x == 0) return;
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, cur); break;
-    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }
 
-void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const TileTask *tasks_dev, int ntasks,
-                          const Cursors &cur, hipStream_t stream)
+void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
+                          hipStream_t stream)
 {
+    const dim3 grid(launch_map_tiles(map));
+    if (grid.x == 0) return;
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), dim3(ntasks), dim3(512), 0, stream, slots_dev, tasks_dev, cur); break;
-    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), dim3(ntasks), dim3(1024), 0, stream, slots_dev, tasks_dev, cur); break;
-    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), dim3(ntasks), dim3(256), 0, stream, slots_dev, tasks_dev, cur); break;
+    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }
 

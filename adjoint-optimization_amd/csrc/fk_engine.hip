@@ -210,13 +210,18 @@ void fill_models(fkadj_instance *I)
     b.ghc = (float)a.ghc;
 }
 
-// Built-in tile table, from measurements on MI355X (grids 256^2..1024^2 at about 4M cells per
-// launch, and single 512^2 instances): 64x16 cells per 512-thread workgroup (two tile cells per
-// thread) is fastest for both kernels; 64x32/512 is within 2% for the forward step and 35% slower
-// for the adjoint step (register pressure), 64x16/256 and 64x32/1024 are slower for both.  The
-// forward tile does not change any result, so a group may pick it from its total size.
-int default_adj_cfg(int, int) { return TILE_16_512; }
-int default_fwd_cfg(int64_t) { return TILE_16_512; }
+// Built-in tile table, from measurements on MI355X (grids 128^2..1024^2 batched to about 4M cells
+// per launch, and single instances of 256^2..1024^2):
+//  * batched launches: 64x16 cells per 512-thread workgroup (two tile cells per thread) is within
+//    1% of the best configuration for both kernels at every size (64x32/512 is 35% slower for the
+//    adjoint step: register pressure);
+//  * launches of fewer than 1M cells leave the GPU part empty with 64x16 tiles: 64x8 cells per
+//    512 threads doubles the workgroups (single 256^2 instance: forward and adjoint steps 20% faster).
+// The forward tile does not change any result, so it follows the launch's size.  The adjoint tile
+// regroups the z.gh partial sums, so it is fixed per instance (by its own size): batched or alone,
+// an instance gives the same result.
+int default_adj_cfg(int rows, int pitch) { return (int64_t)rows * pitch <= 65536 ? TILE_8_512 : TILE_16_512; }
+int default_fwd_cfg(int64_t launch_cells) { return launch_cells < ((int64_t)1 << 20) ? TILE_8_512 : TILE_16_512; }
 
 int validate_and_codes(const fkadj_problem *p, int pitch, std::vector<uint16_t> &code, std::vector<int32_t> &cell_of,
                        int &n_num)
@@ -565,7 +570,10 @@ void group_free(FusedGroupDev *g)
     delete g;
 }
 
-int fwd_cfg_of(const fkadj_instance *I) { return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(0); }
+int fwd_cfg_of(const fkadj_instance *I, int64_t launch_cells)
+{
+    return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(launch_cells);
+}
 
 int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
 {
@@ -578,10 +586,12 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
         launch_map_clear(g->fwd[c]);
         launch_map_clear(g->adj[c]);
     }
+    int64_t cells = 0;
+    for (auto *C : members) cells += (int64_t)C->I->rows * C->I->pitch;
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k]->I;
         slots.push_back(members[k]->slot_h);
-        const int fc = fwd_cfg_of(I);
+        const int fc = fwd_cfg_of(I, cells);
         launch_map_add(g->fwd[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
         launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
     }
@@ -748,7 +758,8 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t st = 3 * n * esz;
     const int64_t nck = (p->nt + K - 1) / K + 1;
     int64_t states = 1 + nck + (K + 1) + 2 + 2 + (p->precision == FKADJ_F64 ? 6 : 2 * K + 1);
-    int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : (pitch / 64) * ((p->nrows + 15) / 16);
+    const int acfg = p->adj_tile > 0 && p->adj_tile < NUM_TILE_CFG ? p->adj_tile : default_adj_cfg(p->nrows, pitch);
+    int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : tiles_of(p->nrows, pitch, acfg);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
                     8 * 3 * n;
     return states * st + extra;
@@ -1424,11 +1435,15 @@ int server_round(fkadj_server *S)
         launch_map_clear(rm[c]);
     }
     for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
+    int64_t fcells = 0;   // cells of this round's forward-kind launch
+    for (int k = 0; k < S->max_slots; k++)
+        if (S->slot_req[k] && S->slot_req[k]->phase != PH_ADJ)
+            fcells += (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];
         if (!r) continue;
         const fkadj_instance *I = r->I;
-        const int K = I->K, f = fwd_cfg_of(I);
+        const int K = I->K, f = fwd_cfg_of(I, fcells);
         if (r->phase == PH_ADJ) {
             ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
             launch_map_add(am[I->adj_cfg], k, tiles_of(I->rows, I->pitch, I->adj_cfg));
@@ -1450,7 +1465,8 @@ int server_round(fkadj_server *S)
             if (++r->step == nt) {
                 terminal<float>(r->C, S->stream);               // objective, z(T), z.gh at T
                 rc_rhs.c[k] = SlotCursor{MODE_RHS, nt - 1, 0, 0};
-                launch_map_add(rm[fwd_cfg_of(I)], k, tiles_of(I->rows, I->pitch, fwd_cfg_of(I)));
+                const int fr = fwd_cfg_of(I, (int64_t)I->rows * I->pitch);
+                launch_map_add(rm[fr], k, tiles_of(I->rows, I->pitch, fr));
                 r->phase = PH_REC;
                 r->seg = (nt + K - 1) / K - 1;
                 r->step = 0;

@@ -66,10 +66,15 @@ void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
 // ---- fused FP32 path: one launch per time step over a tile work-list of one or more instances
 constexpr int FUSED_TX = 64;    // tile width: one wavefront row
 constexpr int FUSED_HALO = 4;   // one cell per RK stage
-// tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512, 4 = 16 x 512, 5 = 32 x 1024
-constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3, TILE_16_512 = 4, TILE_32_1024 = 5;
-constexpr int NUM_TILE_CFG = 6;
-__host__ __device__ constexpr int tile_rows_of(int cfg) { return (cfg == TILE_16_256 || cfg == TILE_16_512) ? 16 : 32; }
+// tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512, 4 = 16 x 512, 5 = 32 x 1024,
+// 6 = 8 x 512, 7 = 8 x 256
+constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3, TILE_16_512 = 4, TILE_32_1024 = 5, TILE_8_512 = 6,
+              TILE_8_256 = 7;
+constexpr int NUM_TILE_CFG = 8;
+__host__ __device__ constexpr int tile_rows_of(int cfg)
+{
+    return (cfg == TILE_16_256 || cfg == TILE_16_512) ? 16 : ((cfg == TILE_8_512 || cfg == TILE_8_256) ? 8 : 32);
+}
 
 struct FusedSlot {
     int rows, pitch, n;

@@ -563,7 +563,7 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 
 // ------------------------------------------------------------------------------ launchers
 // Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
-// 4: 64x16 / 512, 5: 64x32 / 1024.
+// 4: 64x16 / 512, 5: 64x32 / 1024, 6: 64x8 / 512, 7: 64x8 / 256.
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream)
 {
@@ -574,6 +574,8 @@ void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
     case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
     case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
     case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    case 6: hipLaunchKernelGGL((k_fused_fwd<8, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 7: hipLaunchKernelGGL((k_fused_fwd<8, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }
@@ -588,6 +590,8 @@ void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
     case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
     case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
     case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    case 6: hipLaunchKernelGGL((k_fused_adj<8, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 7: hipLaunchKernelGGL((k_fused_adj<8, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     default: hipLaunchKernelGGL((k_fused_adj<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }

@@ -1,7 +1,8 @@
 """Tile autotuner and profiling driver for the fused FP32 kernels.
 
 The fused kernels come in tile configurations (64 x TY cells per workgroup of NT threads):
-1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024.  The forward tile never
+1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024, 6 = 64x8/512,
+7 = 64x8/256.  The forward tile never
 changes a result; the adjoint tile regroups the per-tile z.gh partial sums (rounding-level changes),
 so it is fixed per grid size.  Time blocking is one time step per launch: the kernels are bound by
 vector-ALU issue, not by HBM, so deeper temporal blocking would only add halo recomputation.
@@ -31,8 +32,8 @@ from . import workloads as W
 from .api import batch_grad
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
-FWD_CFGS = (1, 2, 3, 4, 5)
-ADJ_CFGS = (1, 3, 4, 5)
+FWD_CFGS = (1, 2, 3, 4, 5, 6, 7)
+ADJ_CFGS = (1, 3, 4, 5, 6, 7)
 
 
 def synthetic_batch(N: int, B: int, T: float, fwd_tile: int, adj_tile: int, holes: int | None = None):
@@ -122,7 +123,7 @@ def main():
     sizes = args.sizes or [args.grid]
     t = load_table(args.table)
     for N in sizes:
-        B = args.batch if not args.sizes else max(1, round(args.cells / (N * N)))
+        B = args.batch if not args.sizes else min(64, max(1, round(args.cells / (N * N))))   # <= 64 per launch
         t[str(N)] = sweep(N, B, args.T)
     if args.write:
         with open(args.table, "w") as f:

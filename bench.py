@@ -181,7 +181,8 @@ def run_c5(args):
     log(f"[c5] four-spiral state ready in {time.perf_counter() - t0:.1f} s ({ps} phase singularities)")
     A = W.amplitudes(K_amp, seed)
     from fkadj import autotune
-    ft, at = autotune.lookup(N)
+    _, at = autotune.lookup(N)
+    ft = 0
     inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=ft, adj_tile=at)
     inst.set_state(y0)
     for _ in range(args.warmup):
@@ -275,7 +276,8 @@ def run_c3(args):
     from fkadj import autotune
     insts = []
     for m, y in zip(masks, states):
-        ft, at = autotune.lookup(m.shape[0])       # tiles tuned per grid size
+        _, at = autotune.lookup(m.shape[0])        # adjoint tile tuned per grid size
+        ft = 0                                      # forward tile: built-in rule by launch size
         inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at)
         inst.set_state(y)
         insts.append(inst)
@@ -353,7 +355,8 @@ def run_c4(args):
     from fkadj import autotune
     insts = []
     for m, y, T in zip(masks, states, Ts):
-        ft, at = autotune.lookup(m.shape[0])
+        _, at = autotune.lookup(m.shape[0])
+        ft = 0
         inst = fkadj.Instance(fkadj.geometry.active_index_from_mask(m), m.shape[0], m.shape[1], dt=W.DT_MS,
                               nt=int(round(T / W.DT_MS)), model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE,
                               n_amp=T, alpha=2 * W.LAMBDA, gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()),
@@ -451,7 +454,8 @@ def main():
     log(f"[bench] states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
     from fkadj import autotune
-    fwd_tile, adj_tile = autotune.lookup(N)
+    _, adj_tile = autotune.lookup(N)   # the forward tile follows each launch's size (built-in rule)
+    fwd_tile = 0
     insts = []
     for m, y in zip(masks, states):
         inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)

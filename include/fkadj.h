@@ -101,7 +101,8 @@ typedef struct fkadj_problem {
     int precision;                    /* FKADJ_F64 / FKADJ_F32 */
     int ckpt;                         /* forward checkpoint interval K (steps); 0 -> 50 */
     int fwd_tile;                     /* FP32 forward tile: 1 = 64x16 cells / 256 threads, 2 = 64x32 / 256,
-                                         3 = 64x32 / 512, 4 = 64x16 / 512, 5 = 64x32 / 1024;
+                                         3 = 64x32 / 512, 4 = 64x16 / 512, 5 = 64x32 / 1024,
+                                         6 = 64x8 / 512, 7 = 64x8 / 256;
                                          0 -> built-in choice by the cells of the launch (the
                                          forward tile never changes a result) */
     int adj_tile;                     /* FP32 adjoint tile, same ids; 0 -> built-in table */

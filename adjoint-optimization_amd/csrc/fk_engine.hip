@@ -204,6 +204,7 @@ void fill_models(fkadj_instance *I)
     b.i_tau_vp = (float)(1.0 / md.tau_v_plus);
     b.i_tau_wm = (float)(1.0 / md.tau_w_minus);
     b.i_tau_wp = (float)(1.0 / md.tau_w_plus);
+    b.d_tau_w = (float)(1.0 / md.tau_w_plus - 1.0 / md.tau_w_minus);
     b.tau_v2m = (float)md.tau_v2_minus;
     b.dtv = (float)(md.tau_v1_minus - md.tau_v2_minus);
     for (int v = 0; v < 3; v++) b.lc[v] = (float)a.lc[v];

@@ -43,6 +43,7 @@ struct Model32 {
     float two_ksm, two_k;
     float u_c, one_plus_uc;
     float i_tau_d, i_tau_0, i_tau_r, i_tau_si, i_tau_vp, i_tau_wm, i_tau_wp;
+    float d_tau_w;       // 1/tau_w+ - 1/tau_w-
     float tau_v2m, dtv;  // tau_v- = tau_v2m + q*dtv
     float lc[3];
     float ghc;
@@ -129,13 +130,20 @@ __device__ inline Gates32 gates32(const Model32 &m, float u)
     return g;
 }
 
+// The reaction written with the fewest operations (FP32 only; the FP64 path keeps the reference's
+// form):  (1-u)(u-u_c) = u (1+u_c-u) - u_c,  J_fi + J_so terms grouped by p, and
+// (1-p)(1-v)/tau_v- - p v/tau_v+ = a - v (a + b) with a = (1-p)/tau_v-, b = p/tau_v+ (same for w).
 __device__ inline void fion32(const Model32 &m, const Gates32 &g, float u, float v, float w, float &fu,
                               float &fv, float &fw)
 {
-    float gg = (1.0f - u) * (u - m.u_c);
-    fu = v * g.p * gg * m.i_tau_d - u * g.omp * m.i_tau_0 - g.p * m.i_tau_r + w * g.s * m.i_tau_si;
-    fv = g.omp * (1.0f - v) * g.itvm - g.p * v * m.i_tau_vp;
-    fw = g.omp * (1.0f - w) * m.i_tau_wm - g.p * w * m.i_tau_wp;
+    const float gg = u * (m.one_plus_uc - u) - m.u_c;
+    const float a = fmaf(v * gg, m.i_tau_d, -m.i_tau_r);
+    const float b = fmaf(w * g.s, m.i_tau_si, -(u * g.omp) * m.i_tau_0);
+    fu = fmaf(g.p, a, b);
+    const float av = g.omp * g.itvm, bv = g.p * m.i_tau_vp;
+    fv = fmaf(-v, av + bv, av);
+    const float aw = g.omp * m.i_tau_wm, bw = g.p * m.i_tau_wp;
+    fw = fmaf(-w, aw + bw, aw);
 }
 
 // Transposed Jacobian entries (seven non-zeros) at one state.
@@ -143,23 +151,25 @@ struct Jac32 {
     float a00, a01, a02, a10, a11, a20, a22;
 };
 
+// Same grouping as fion32: d(1/tau_v-)/du = -(1/tau_v-)^2 dq (tau_v1- - tau_v2-) enters a01 as
+// -(1-v)(1/tau_v-)(dp + (1-p)(1/tau_v-) dq dtv).
 __device__ inline Jac32 jac32(const Model32 &m, const Gates32 &g, float u, float v, float w)
 {
     Jac32 J;
-    float dp = m.two_ksm * g.p * g.omp;
-    float dq = m.two_ksm * g.q * g.omq;
-    float ds = m.two_k * g.s * g.oms;
-    float gg = (1.0f - u) * (u - m.u_c);
-    float dg = m.one_plus_uc - 2.0f * u;
-    float itvm = g.itvm;
-    J.a00 = v * (dp * gg + g.p * dg) * m.i_tau_d - (g.omp - u * dp) * m.i_tau_0 - dp * m.i_tau_r
-          + w * ds * m.i_tau_si;
+    const float dp = m.two_ksm * g.p * g.omp;
+    const float dq = m.two_ksm * g.q * g.omq;
+    const float ds = m.two_k * g.s * g.oms;
+    const float gg = u * (m.one_plus_uc - u) - m.u_c;
+    const float dg = m.one_plus_uc - 2.0f * u;
+    const float av = g.omp * g.itvm, bv = g.p * m.i_tau_vp;
+    J.a00 = fmaf(v * m.i_tau_d, fmaf(dp, gg, g.p * dg),
+                 fmaf(fmaf(u, dp, -g.omp), m.i_tau_0, fmaf(w * ds, m.i_tau_si, -dp * m.i_tau_r)));
     J.a10 = g.p * gg * m.i_tau_d;
     J.a20 = g.s * m.i_tau_si;
-    J.a01 = -dp * (1.0f - v) * itvm - g.omp * (1.0f - v) * dq * m.dtv * itvm * itvm - dp * v * m.i_tau_vp;
-    J.a11 = -g.omp * itvm - g.p * m.i_tau_vp;
-    J.a02 = -dp * ((1.0f - w) * m.i_tau_wm + w * m.i_tau_wp);
-    J.a22 = -g.omp * m.i_tau_wm - g.p * m.i_tau_wp;
+    J.a01 = -fmaf((1.0f - v) * g.itvm, fmaf(av * dq, m.dtv, dp), dp * v * m.i_tau_vp);
+    J.a11 = -(av + bv);
+    J.a02 = -dp * fmaf(w, m.d_tau_w, m.i_tau_wm);
+    J.a22 = -fmaf(g.omp, m.i_tau_wm, g.p * m.i_tau_wp);
     return J;
 }
 

@@ -345,7 +345,7 @@ def run_c4(args):
     import fkadj
     from fkadj import workloads as W
     rng = np.random.default_rng(4)
-    P, R, rate, budget = args.c4_instances, args.c4_requests, 200.0, 200 * 10 ** 9
+    P, R, rate, budget = args.c4_instances, args.c4_requests, args.c4_rate, 200 * 10 ** 9
     sizes = [int(x) for x in rng.choice(np.arange(256, 1025, 64), P)]
     Ts = [int(x) for x in rng.integers(50, 201, P)]
     seeds = list(range(200, 200 + P))
@@ -393,7 +393,7 @@ def run_c4(args):
     lat = np.array([1e3 * (done_t[r] - arrivals[r]) for r in range(R)])
     same = bool(np.array_equal(results[0][0], ref0[0]) and results[0][1] == ref0[1])
     cell_steps = sum(insts[w].nrows * ((insts[w].ncols + 63) // 64 * 64) * insts[w].nt for w in which)
-    out = {"metric": "served gradient evals/s (Poisson 200 req/s) with p50/p99 latency", "value": R / makespan,
+    out = {"metric": f"served gradient evals/s (Poisson {rate:g} req/s) with p50/p99 latency", "value": R / makespan,
            "unit": "evals/s", "n_gpus": 1, "steps": R, "warmup": 1, "ms_per_step": 1e3 * makespan / R,
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"C4: {R} requests on {P} resident instances, grids 256..1024 step 64, "
@@ -424,6 +424,7 @@ def main():
                          "grids; c4: served Poisson stream; c5: 2048^2, 25 000 steps")
     ap.add_argument("--c4-instances", type=int, default=24)
     ap.add_argument("--c4-requests", type=int, default=96)
+    ap.add_argument("--c4-rate", type=float, default=200.0, help="offered load of the C4 stream (req/s)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched instances of this "

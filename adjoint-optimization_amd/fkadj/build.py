@@ -58,5 +58,20 @@ def build_copy_probe() -> str:
     return build_probe("copy_probe")
 
 
+def build_c_client() -> str:
+    """tests/c_abi_client.c, a plain C99 caller of the C ABI (gcc; rpath to this directory)."""
+    root = os.path.dirname(os.path.dirname(PKG))
+    src = os.path.join(root, "tests", "c_abi_client.c")
+    out_dir = os.path.join(root, "tests", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "c_abi_client")
+    cmd = ["gcc", "-std=c99", "-O2", "-Wall", "-Wextra", "-Werror", "-I", os.path.join(root, "include"), src,
+           "-L", PKG, "-lfkadj", "-Wl,-rpath,$ORIGIN/../../adjoint-optimization_amd/fkadj", "-lm", "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"gcc failed for {src}:\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

@@ -228,8 +228,9 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 }
 
 // ------------------------------------------------------------------------------ forward
-template <int TY, int NT>
-__global__ __launch_bounds__(NT) void k_fused_fwd(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
+// WPE: the occupancy asked of the register allocator (waves per SIMD; 1 = no request).
+template <int TY, int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT;
@@ -356,8 +357,8 @@ __device__ inline T h4p(T x0, T x1, T m0, T m1)
            T(-s * s * (6.0 - 16.0 * s + 9.0 * s * s)) * x1 + T((s - 1.0) * (9.0 * s - 5.0) * s * s / 4.0) * m1;
 }
 
-template <int TY, int NT>
-__global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
+template <int TY, int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
@@ -563,20 +564,24 @@ __global__ __launch_bounds__(NT) void k_fused_adj(const FusedSlot *__restrict__ 
 
 // ------------------------------------------------------------------------------ launchers
 // Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
-// 4: 64x16 / 512, 5: 64x32 / 1024, 6: 64x8 / 512, 7: 64x8 / 256.
+// 4: 64x16 / 512, 5: 64x32 / 1024, 6: 64x8 / 512, 7: 64x8 / 256.  The occupancy requests (third
+// template argument) are measured choices: the 64x16/512 forward step at 8 waves per SIMD (64
+// VGPRs, two spilled dwords) runs 8% faster than at the 6 its 67 VGPRs allow; the 64x8/512 adjoint
+// step fits 6 waves without spilling.  Requests that force larger spills (64x16/512 adjoint at 6
+// waves: 190 us instead of 120 us per 16 x 512^2 launch) are not made.
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream)
 {
     const dim3 grid(launch_map_tiles(map));
     if (grid.x == 0) return;
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
-    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
-    case 6: hipLaunchKernelGGL((k_fused_fwd<8, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 7: hipLaunchKernelGGL((k_fused_fwd<8, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
-    default: hipLaunchKernelGGL((k_fused_fwd<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 2: hipLaunchKernelGGL((k_fused_fwd<32, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 3: hipLaunchKernelGGL((k_fused_fwd<32, 512, 1>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 4: hipLaunchKernelGGL((k_fused_fwd<16, 512, 8>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 5: hipLaunchKernelGGL((k_fused_fwd<32, 1024, 6>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    case 6: hipLaunchKernelGGL((k_fused_fwd<8, 512, 1>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 7: hipLaunchKernelGGL((k_fused_fwd<8, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    default: hipLaunchKernelGGL((k_fused_fwd<16, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }
 
@@ -586,13 +591,13 @@ void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
     const dim3 grid(launch_map_tiles(map));
     if (grid.x == 0) return;
     switch (cfg) {
-    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
-    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
-    case 6: hipLaunchKernelGGL((k_fused_adj<8, 512>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
-    case 7: hipLaunchKernelGGL((k_fused_adj<8, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
-    default: hipLaunchKernelGGL((k_fused_adj<16, 256>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 2: hipLaunchKernelGGL((k_fused_adj<32, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    case 3: hipLaunchKernelGGL((k_fused_adj<32, 512, 1>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 4: hipLaunchKernelGGL((k_fused_adj<16, 512, 1>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 5: hipLaunchKernelGGL((k_fused_adj<32, 1024, 1>), grid, dim3(1024), 0, stream, slots_dev, cur, map); break;
+    case 6: hipLaunchKernelGGL((k_fused_adj<8, 512, 6>), grid, dim3(512), 0, stream, slots_dev, cur, map); break;
+    case 7: hipLaunchKernelGGL((k_fused_adj<8, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
+    default: hipLaunchKernelGGL((k_fused_adj<16, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
 }
 

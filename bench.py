@@ -41,9 +41,21 @@ def algorithmic_bytes_per_cell_step(K: int, w: int = 4) -> float:
     return w * (7 * (1 + R) + 13)
 
 
-def cpu_baseline(mask, y0, A, nt_full, nt_sample=100):
-    """FP64 oracle (C, OpenMP) on the host: a bounded sample of the same workload (the first
-    nt_sample steps of instance 0, all three sweeps), scaled to a full evaluation."""
+def _cpu_model() -> str:
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(mask, y0, A, nt_full, nt_sample=100, nt_single=10):
+    """FP64 oracle (C, OpenMP, -O3 AVX2) on the host: a bounded sample of the same workload (the
+    first nt_sample steps of instance 0, all three sweeps) on all host threads, scaled to a full
+    evaluation; and the first nt_single steps on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from fkadj import geometry as G
@@ -55,17 +67,26 @@ def cpu_baseline(mask, y0, A, nt_full, nt_sample=100):
     gh = np.full(mask.size, np.nan)
     gh[ai > 0] = om.Du * 2 / om.h * G.ghost_lr_from_mask(mask)
     P = nt_full // len(A)
-    Ef = np.repeat(A, 3 * P)[: 3 * nt_sample]
     S = G.mean_u2_S(int(mask.sum()))
-    t0 = time.perf_counter()
-    O.grad(om, planes, gh.reshape(mask.shape), W.DT_MS, Ef, Ef.copy(), ai, G.csc_arrays(S), (1, 0, 0), ckpt=50)
-    dt = time.perf_counter() - t0
+
+    def timed(nsteps):
+        Ef = np.repeat(A, 3 * P)[: 3 * nsteps]
+        t0 = time.perf_counter()
+        O.grad(om, planes, gh.reshape(mask.shape), W.DT_MS, Ef, Ef.copy(), ai, G.csc_arrays(S), (1, 0, 0), ckpt=50)
+        return time.perf_counter() - t0
+
+    threads = O.max_threads()
+    dt = timed(nt_sample)
+    O.set_threads(1)
+    dt1 = timed(nt_single)
+    O.set_threads(threads)
     per_eval = dt * nt_full / nt_sample
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
     return {"value": 1.0 / per_eval, "unit": "evals/s", "cores": threads, "kind": "port",
             "sample": f"FP64 oracle, instance 0 (512^2), first {nt_sample} of {nt_full} steps with forward, "
                       f"recompute and adjoint sweeps ({dt:.2f} s), scaled x{nt_full // nt_sample}",
-            "seconds_per_eval": per_eval}
+            "seconds_per_eval": per_eval, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+            "single_thread": {"seconds_per_eval": dt1 * nt_full / nt_single,
+                              "sample": f"first {nt_single} steps ({dt1:.2f} s) on 1 thread"}}
 
 
 def oracle_piecewise(mask, y0, A, nt):
@@ -146,7 +167,9 @@ def run_c1(args):
     g_cpu, Lc_cpu, sec = oracle_piecewise(m, y0, A, nt)
     log(f"[c1] FP64 oracle on the host: {sec:.1f} s/eval")
     g64, g32 = res["plain_f64"][0][0], res["fused_f32"][0][0]
-    threads = int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    threads = O.max_threads()
     out = {"metric": "gradient evals/s, C1 128^2 FP64 (plain GPU path; CPU oracle beside)",
            "value": 1.0 / res["plain_f64"][1], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * res["plain_f64"][1], "higher_is_better": True,

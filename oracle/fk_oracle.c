@@ -25,6 +25,7 @@
  * Build: gcc -O2 -fopenmp -ffp-contract=off -fPIC -shared (no fast-math: NaN semantics matter).
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -35,6 +36,10 @@ typedef struct {
 } orc_model;
 
 /* Constants of src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24 (h is a run-time value here). */
+/* Host threads used by the OpenMP loops below (the CPU-baseline timing runs all cores and one). */
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : 1); }
+int orc_max_threads(void) { return omp_get_max_threads(); }
+
 void orc_model_reference(orc_model *m)
 {
     m->u_c = 0.13; m->u_v = 0.04; m->tau_d = 0.388; m->tau_v1_minus = 7.6; m->tau_v2_minus = 8.8;

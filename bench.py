@@ -24,10 +24,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell of one launch of the fused kernels (64x16/512 tiles), measured with rocprofv3
+# HBM bytes per cell of one launch of the fused kernels (forward 64x16/512, adjoint 64x8/256 tiles, the
+# benchmark's launch shapes), measured with rocprofv3
 # --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2 batch and scaled by the factors a 1 GiB copy kernel
 # gives those counters (x2.0 and x1.0); forward is the mean of forward-sweep and recompute launches
-HBM_BYTES_PER_CELL = {"adj": 75.5, "fwd": 32.2}
+HBM_BYTES_PER_CELL = {"adj": 74.1, "fwd": 34.8}
 
 
 def log(*a):

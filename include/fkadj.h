@@ -8,7 +8,8 @@
  *
  * Reference interfaces replaced (paths relative to the reference checkout):
  *   fk_grad_Et_comp          src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1572  -> fkadj_grad_ref
- *   mexFunction (gradient)   src/cuda/fk_grad_Et_param_mex.cpp:11-92           -> INTEGRATION.md stub
+ *   mexFunction (gradient)   src/cuda/fk_grad_Et_param_mex.cpp:11-92           -> a MEX gateway over
+ *                                                                                 fkadj_grad_ref
  *   fk_forward_Et_comp       src/cuda/fk_forward_Et_mono_PARAMETERS.cu:646     -> fkadj_forward
  *   compile-time constants   src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24 -> fkadj_model
  *   parameters_string        src/cuda/fk_parameters_string_00_05_alpha_0p8_dx_0p035.cpp:1-32

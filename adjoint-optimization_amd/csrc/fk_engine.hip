@@ -150,6 +150,7 @@ struct EvalCtx {
     double *d_zgh = nullptr, *d_L0part = nullptr, *d_L0 = nullptr;
     char *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
     char *d_kseg = nullptr, *d_kT = nullptr;          // FP32: stored slopes of the recompute sweep
+    char *d_kfirst = nullptr;                         // FP32: first slope of each segment, mod 3
     FusedSlot slot_h{};
     struct FusedGroupDev *group = nullptr;            // this context alone (FP32)
     std::vector<double> Ef, Ea;                       // host tables of the last evaluation
@@ -161,6 +162,10 @@ struct EvalCtx {
 struct FusedGroupDev {
     std::vector<EvalCtx *> members;
     FusedSlot *d_slots = nullptr;
+    // the adjoint steps of segment s run on `side` while the caller's stream recomputes segment
+    // s-1; ev_rec[p] / ev_adj[p]: recompute / adjoint of the last segment of parity p done
+    hipStream_t side = nullptr;
+    hipEvent_t ev_rec[2] = {}, ev_adj[2] = {};
     // one launch map per kernel configuration (index = tile config id); slot k = member k
     LaunchMap fwd[NUM_TILE_CFG], adj[NUM_TILE_CFG];
 };
@@ -442,7 +447,10 @@ struct Sampler {
 int64_t ctx_bytes(const fkadj_instance *I)
 {
     const int64_t st = (int64_t)I->st_bytes;
-    int64_t states = I->nck + (I->K + 1) + 2 + 2 + (I->prec == FKADJ_F64 ? 6 : 2 * I->K + 1);
+    // FP64: checkpoints, one segment, ring, adjoint pair, 6 stage planes; FP32: checkpoints, two
+    // segments, ring, adjoint pair, 2K stored slopes, f(y_nt), 3 first slopes
+    int64_t states = I->prec == FKADJ_F64 ? I->nck + (I->K + 1) + 2 + 2 + 6
+                                          : I->nck + 2 * (I->K + 1) + 2 + 2 + 2 * I->K + 1 + 3;
     int64_t tables = 8 * (int64_t)(6 * I->nt + (int64_t)I->nt * I->nb_part + I->nb_plain + (I->nt + 1) +
                                    (I->n_num + 255) / 256 + 1);
     return states * st + tables;
@@ -451,7 +459,7 @@ int64_t ctx_bytes(const fkadj_instance *I)
 void ctx_free(EvalCtx *C)
 {
     if (!C) return;
-    for (char **q : {&C->d_ck, &C->d_seg, &C->d_ring, &C->d_z, &C->d_work, &C->d_kseg, &C->d_kT})
+    for (char **q : {&C->d_ck, &C->d_seg, &C->d_ring, &C->d_z, &C->d_work, &C->d_kseg, &C->d_kT, &C->d_kfirst})
         if (*q) (void)hipFree(*q);
     for (double **q : {&C->d_Ef, &C->d_Ea, &C->d_part, &C->d_partT, &C->d_zgh, &C->d_L0part, &C->d_L0})
         if (*q) (void)hipFree(*q);
@@ -459,8 +467,10 @@ void ctx_free(EvalCtx *C)
     delete C;
 }
 
-// Allocates a zeroed context on stream s (hole cells read 0 in every state buffer).
-int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out)
+// Allocates a zeroed context on stream s (hole cells read 0 in every state buffer).  An FP32
+// context also gets its own one-member group (launch maps, a second stream) unless it is only
+// driven through a server (self_group false).
+int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group = true)
 {
     auto *C = new (std::nothrow) EvalCtx();
     if (!C) return FKADJ_ERR_NOMEM;
@@ -476,10 +486,11 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out)
     const int nt = I->nt;
     bool ok = ald(C->d_Ef, 3 * (size_t)nt) && ald(C->d_Ea, 3 * (size_t)nt) && ald(C->d_part, (size_t)nt * I->nb_part) &&
               ald(C->d_partT, I->nb_plain) && ald(C->d_zgh, nt + 1) && ald(C->d_L0part, objective_blocks(I->n_num)) &&
-              ald(C->d_L0, 1) && alc(C->d_ck, st * I->nck) && alc(C->d_seg, st * (I->K + 1)) &&
+              ald(C->d_L0, 1) && alc(C->d_ck, st * I->nck) &&
+              alc(C->d_seg, st * (I->K + 1) * (I->prec == FKADJ_F32 ? 2 : 1)) &&
               alc(C->d_ring, st * 2) && alc(C->d_z, st * 2);
     if (ok && I->prec == FKADJ_F64) ok = alc(C->d_work, st * 6);
-    if (ok && I->prec == FKADJ_F32) ok = alc(C->d_kseg, st * 2 * I->K) && alc(C->d_kT, st);
+    if (ok && I->prec == FKADJ_F32) ok = alc(C->d_kseg, st * 2 * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
     if (ok && I->prec == FKADJ_F32) {
         FusedSlot &S = C->slot_h;
         S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
@@ -487,11 +498,13 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out)
         S.ntiles = I->nb_part;
         S.code = I->d_code;
         S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
-        S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT);
+        S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT); S.kfirst = as<float>(C->d_kfirst);
         S.Ef = C->d_Ef; S.Ea = C->d_Ea; S.partial = C->d_part;
         S.m = I->m32; S.dt = (float)I->p.dt;
-        std::vector<EvalCtx *> self{C};
-        ok = group_build(self, s, &C->group) == FKADJ_OK;
+        if (self_group) {
+            std::vector<EvalCtx *> self{C};
+            ok = group_build(self, s, &C->group) == FKADJ_OK;
+        }
     }
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) {
@@ -567,7 +580,13 @@ int run_plain(EvalCtx *C, Sampler &smp)
 void group_free(FusedGroupDev *g)
 {
     if (!g) return;
+    if (g->side) (void)hipStreamSynchronize(g->side);
     if (g->d_slots) (void)hipFree(g->d_slots);
+    for (int p = 0; p < 2; p++) {
+        if (g->ev_rec[p]) (void)hipEventDestroy(g->ev_rec[p]);
+        if (g->ev_adj[p]) (void)hipEventDestroy(g->ev_adj[p]);
+    }
+    if (g->side) (void)hipStreamDestroy(g->side);
     delete g;
 }
 
@@ -598,6 +617,10 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
     }
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) == hipSuccess;
+    for (int p = 0; p < 2 && ok; p++)
+        ok = hipEventCreateWithFlags(&g->ev_rec[p], hipEventDisableTiming) == hipSuccess &&
+             hipEventCreateWithFlags(&g->ev_adj[p], hipEventDisableTiming) == hipSuccess;
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) { group_free(g); return FKADJ_ERR_NOMEM; }
     *out = g;
@@ -633,14 +656,28 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     HIPCHK(hipGetLastError());
     for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step
+    // Backward pass, two streams: the recompute of segment s-1 (this stream) runs while the adjoint
+    // steps of segment s run on the side stream.  Segments alternate between two state / slope
+    // buffers, so the recompute of segment s-2 waits for the adjoint of segment s.  Launches of
+    // fewer than 2M cells leave the GPU part idle and gain from the overlap (single 512^2 instance:
+    // evaluation 325 -> 263 ms; 256^2: 238 -> 181 ms); a full GPU does not (16 x 512^2: 1% slower),
+    // so larger groups keep both sweeps on one stream (the events then order nothing new).
     const int nseg = (nt + K - 1) / K;
+    int64_t cells = 0;
+    for (auto *C : g->members) cells += (int64_t)C->I->rows * C->I->pitch;
+    hipStream_t side = cells < ((int64_t)2 << 20) ? g->side : s;
     for (int sg = nseg - 1; sg >= 0; sg--) {
-        const int a = sg * K, b = std::min(a + K, nt);
+        const int a = sg * K, b = std::min(a + K, nt), p = sg & 1;
+        if (sg + 2 <= nseg - 1) HIPCHK(hipStreamWaitEvent(s, g->ev_adj[p], 0));
         // all b-a steps are replayed: the last one only to store the slope f(y_{b-1})
         for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
-        for (int i = b; i > a; i--) smp.adj_launch(s, [&] { group_adjoint(g, i, a, s); });
+        HIPCHK(hipEventRecord(g->ev_rec[p], s));
+        HIPCHK(hipStreamWaitEvent(side, g->ev_rec[p], 0));
+        for (int i = b; i > a; i--) smp.adj_launch(side, [&] { group_adjoint(g, i, a, side); });
+        HIPCHK(hipEventRecord(g->ev_adj[p], side));
         HIPCHK(hipGetLastError());
     }
+    HIPCHK(hipStreamWaitEvent(s, g->ev_adj[0], 0));   // segment 0's adjoint: the last side-stream work
     for (auto *C : g->members) sum_rows_launch(C->d_part, C->I->nb_part, nt, C->d_zgh, s);
     HIPCHK(hipGetLastError());
     return FKADJ_OK;
@@ -758,7 +795,7 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t esz = p->precision == FKADJ_F32 ? 4 : 8;
     const int64_t st = 3 * n * esz;
     const int64_t nck = (p->nt + K - 1) / K + 1;
-    int64_t states = 1 + nck + (K + 1) + 2 + 2 + (p->precision == FKADJ_F64 ? 6 : 2 * K + 1);
+    int64_t states = 1 + (p->precision == FKADJ_F64 ? nck + (K + 1) + 2 + 2 + 6 : nck + 2 * (K + 1) + 2 + 2 + 2 * K + 1 + 3);
     const int acfg = p->adj_tile > 0 && p->adj_tile < NUM_TILE_CFG ? p->adj_tile : default_adj_cfg(p->nrows, pitch);
     int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : tiles_of(p->nrows, pitch, acfg);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
@@ -1390,7 +1427,7 @@ int server_admit(fkadj_server *S)
             continue;
         }
         if (S->used + bytes > S->budget) return FKADJ_OK;
-        int rc = ctx_create(I, S->stream, &r->C);
+        int rc = ctx_create(I, S->stream, &r->C, false);
         if (rc == FKADJ_OK) rc = build_tables(I, r->C, r->E.data(), S->stream);
         if (rc == FKADJ_OK && hipHostMalloc((void **)&r->h_out, sizeof(double) * (I->nt + 2)) != hipSuccess)
             rc = FKADJ_ERR_NOMEM;

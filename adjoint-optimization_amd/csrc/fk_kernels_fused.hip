@@ -247,6 +247,7 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
     const float *yin;
     float *yout;
     float *kout = nullptr;      // stage-1 slopes of the tile (MODE_REC, MODE_RHS)
+    float *kout2 = nullptr;     // second copy of the first slope of a segment (MODE_REC, j = 0)
     float E1, E23, E4;
     if (mode == MODE_FWD) {
         yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
@@ -254,10 +255,16 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
         yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
                                              : S.ring + (size_t)(nx & 1) * st;
     } else if (mode == MODE_REC) {
-        int j = step - seg_a;
-        yin = (j == 0) ? S.ck + (size_t)(seg_a / S.K) * st : S.seg + (size_t)j * st;
-        yout = S.seg + (size_t)(j + 1) * st;
-        kout = S.kseg + ((size_t)((seg_a / S.K) & 1) * S.K + j) * st;
+        // segments alternate between two state buffers and two slope buffers, so the recompute of
+        // segment s-1 may run while the adjoint steps of segment s read segment s's buffers; the
+        // first slope of each segment is also kept by segment index mod 3 for the adjoint step
+        // that ends at it (the last one of the previous segment)
+        const int j = step - seg_a, segi = seg_a / S.K;
+        float *segb = S.seg + (size_t)(segi & 1) * (S.K + 1) * st;
+        yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
+        yout = segb + (size_t)(j + 1) * st;
+        kout = S.kseg + ((size_t)(segi & 1) * S.K + j) * st;
+        if (j == 0) kout2 = S.kfirst + (size_t)(segi % 3) * st;
     } else {
         yin = S.ck + (size_t)(S.nck - 1) * st;
         yout = nullptr;
@@ -310,6 +317,9 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
                 int kk = gy * S.pitch + gx;
 #pragma unroll
                 for (int v = 0; v < 3; v++) gmem(kout)[v * S.n + kk] = f[sl][v];
+                if (kout2)
+#pragma unroll
+                    for (int v = 0; v < 3; v++) gmem(kout2)[v * S.n + kk] = f[sl][v];
             }
             if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
         }
@@ -376,12 +386,13 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     const int segi = seg_a / S.K;
     const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
     const float *kbuf = S.kseg + (size_t)(segi & 1) * S.K * st;
-    const auto *yA = gmem((i - 1 == seg_a) ? S.ck + (size_t)segi * st : S.seg + (size_t)(i - 1 - seg_a) * st);
-    const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : S.seg + (size_t)(i - seg_a) * st);
+    const float *segb = S.seg + (size_t)(segi & 1) * (S.K + 1) * st;
+    const auto *yA = gmem((i - 1 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 1 - seg_a) * st);
+    const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st);
     const auto *kA = gmem(kbuf + (size_t)(i - 1 - seg_a) * st);
     // f(y_i): slope of the next recomputed step, the first slope of the later segment, or f(y_nt)
     const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st
-                                      : (b_end < S.nt ? S.kseg + (size_t)((segi + 1) & 1) * S.K * st : S.kT));
+                                      : (b_end < S.nt ? S.kfirst + (size_t)((segi + 1) % 3) * st : S.kT));
     const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
     auto *zout = gmem(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
     const Model32 m = S.m;

@@ -63,7 +63,7 @@ void plain_adjoint_step(const PlainCtx<T, M> &c, const T *yA, const T *yB, const
 template <typename T, typename M>
 void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
 
-// ---- fused FP32 path: one launch per time step over a tile work-list of one or more instances
+// ---- fused FP32 path: one launch per time step over the tiles of one or more instances
 constexpr int FUSED_TX = 64;    // tile width: one wavefront row
 constexpr int FUSED_HALO = 4;   // one cell per RK stage
 // tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512, 4 = 16 x 512, 5 = 32 x 1024,

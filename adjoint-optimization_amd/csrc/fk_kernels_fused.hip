@@ -1,4 +1,4 @@
-// fk_kernels_fused.hip -- FP32 fused time-step kernels over a tile work-list (one or many instances).
+// fk_kernels_fused.hip -- FP32 fused time-step kernels over the tiles of one or many instances.
 //
 // One launch advances every listed tile by one full time step:
 //   forward  : the four RK4 stages of G:460-563 (G = src/cuda/fk_grad_Et_mono_PARAMETERS.cu)

@@ -147,7 +147,7 @@ int fkadj_stimulate(fkadj_instance *inst, const uint8_t *cell_mask, double value
 /* ---------------------------------------------------------------- batched evaluation */
 /* Gradients of `count` (<= 64) FP32 instances (any grid sizes, equal nt and ckpt) computed together:
  * every forward, recompute and adjoint step of all instances is one kernel launch over a
- * tile work-list of all instances.  Results equal the instances' own fkadj_grad bit for bit. */
+ * launch over the tiles of all instances.  Results equal the instances' own fkadj_grad bit for bit. */
 int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
                      double *Lcost, double *L0, double *LE);
 

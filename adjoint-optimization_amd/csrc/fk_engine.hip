@@ -524,7 +524,8 @@ int ensure_default_ctx(fkadj_instance *I)
 
 // ------------------------------------------------------------------------ one evaluation
 // Forward sweep with checkpoints, objective and terminal adjoint, then per segment (last to
-// first) recompute + adjoint steps.  All launches go to one stream.
+// first) recompute + adjoint steps, on the caller's stream (the FP32 backward pass may also use the
+// group's side stream, joined before the final sums).
 template <typename T>
 void terminal(EvalCtx *C, hipStream_t s)
 {

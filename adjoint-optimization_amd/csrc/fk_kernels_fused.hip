@@ -7,14 +7,18 @@
 // A 64 x TY output tile is staged in LDS with a 4-cell halo (one cell per RK stage); stage k is
 // evaluated on the tile grown by 4-k cells, so the tile needs no neighbour communication inside a
 // step and HBM sees one read of the inputs and one write of the outputs per step.
-// Work split: thread t owns the tile cells t + NT*s (one wavefront = one 64-cell row) and ring
-// cells of the halo: the cells of rings d = 1, 2, 3 around the tile are numbered in one sequence
-// (ring 1 first) and dealt out NT at a time, every other pass in reverse thread order, so each
-// wavefront carries about the same share of halo work (a ring-d cell takes part in stages 1..4-d).
+// Work split: thread t owns the tile cells t + NT*s (one wavefront = one 64-cell row) and halo
+// cells.  Stage k is needed at the cells within Manhattan distance 4-k of the tile (a stage-k value
+// reads the stage-(k-1) values of its four neighbours), so the halo cells computed are the shells
+// m = 1, 2, 3 of cells at distance m: four straight bands beside the tile edges plus m-1 cells per
+// corner (the square rings' far corner cells are never read).  The shells are numbered in one
+// sequence (shell 1 first) and dealt out NT at a time, every other pass in reverse thread order, so
+// each wavefront carries about the same share of halo work (a shell-m cell takes part in stages
+// 1..4-m).
 // The ownership is the same in every stage, so per-cell values (state, RK increments, Jacobians)
 // stay in registers across the stages; LDS only holds the values neighbours read.
 // Multiply-adds are fused only inside one source expression (as written), never across
-// statements: a cell's arithmetic is then the same whichever slot (tile or halo ring) computes it,
+// statements: a cell's arithmetic is then the same whichever slot (tile or halo shell) computes it,
 // so results do not depend on the tile configuration.
 #pragma clang fp contract(on)
 
@@ -29,29 +33,30 @@ struct Tile {
     static constexpr int RXP = RX + 1;               // odd LDS row pitch: column walks hit distinct banks
     static constexpr int TS = TX * TY / NT;          // tile cells per thread
     static_assert(TX * TY % NT == 0, "tile must split evenly");
-    __host__ __device__ static constexpr int ring_count(int d) { return 2 * (TX + 2 * d) + 2 * (TY + 2 * d - 2); }
-    static constexpr int RC = ring_count(1) + ring_count(2) + ring_count(3);   // halo cells computed
-    static constexpr int NRS = (RC + NT - 1) / NT;                               // ring slots per thread
+    // cells at Manhattan distance m from the tile: 2 TX + 2 TY in the bands, m - 1 per corner
+    __host__ __device__ static constexpr int shell_count(int m) { return 2 * (TX + TY) + 4 * (m - 1); }
+    static constexpr int RC = shell_count(1) + shell_count(2) + shell_count(3);   // halo cells computed
+    static constexpr int NRS = (RC + NT - 1) / NT;                                 // halo slots per thread
     static constexpr int NSLOT = TS + NRS;
-    // ring of the r-th halo cell in the ring-1, ring-2, ring-3 sequence
+    // shell of the r-th halo cell in the shell-1, shell-2, shell-3 sequence
     __host__ __device__ static constexpr int depth_of(int r)
     {
-        return r < ring_count(1) ? 1 : (r < ring_count(1) + ring_count(2) ? 2 : 3);
+        return r < shell_count(1) ? 1 : (r < shell_count(1) + shell_count(2) ? 2 : 3);
     }
-    // ring slot j holds sequence numbers NT*j .. NT*(j+1)-1: its smallest ring
+    // halo slot j holds sequence numbers NT*j .. NT*(j+1)-1: its smallest shell
     __host__ __device__ static constexpr int min_depth(int j) { return depth_of(NT * j); }
 };
 
 // Cell of slot `sl` of thread `tid` (LDS coordinates) and the last RK stage it takes part in
-// (4 for tile cells, 4-d for a ring-d cell); returns false if the thread owns no cell in that slot.
+// (4 for tile cells, 4-m for a shell-m cell); returns false if the thread owns no cell in that slot.
 template <class G>
 __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_stage)
 {
-    constexpr int TY = G::RY - 2 * G::H;
+    constexpr int TX = G::TX, TY = G::RY - 2 * G::H, H = G::H;
     if (sl < G::TS) {
         int lin = tid + G::NT * sl;
-        ly = G::H + lin / G::TX;
-        lx = G::H + lin % G::TX;
+        ly = H + lin / TX;
+        lx = H + lin % TX;
         last_stage = 4;
         return true;
     }
@@ -59,29 +64,36 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_st
     int e = (j & 1) ? G::NT * (j + 1) - 1 - tid : G::NT * j + tid;
     last_stage = 0;
     if (e >= G::RC) return false;
-    int d = 1;
-    if (e >= G::ring_count(1)) {
-        e -= G::ring_count(1);
-        d = 2;
-        if (e >= G::ring_count(2)) {
-            e -= G::ring_count(2);
-            d = 3;
+    int m = 1;
+    if (e >= G::shell_count(1)) {
+        e -= G::shell_count(1);
+        m = 2;
+        if (e >= G::shell_count(2)) {
+            e -= G::shell_count(2);
+            m = 3;
         }
     }
-    last_stage = 4 - d;
-    const int W = G::TX + 2 * d, Hh = TY + 2 * d, base = G::H - d;
-    if (e < W) { ly = base; lx = base + e; }
-    else if (e < 2 * W) { ly = G::H + TY + d - 1; lx = base + e - W; }
-    else {
-        int e2 = e - 2 * W;
-        if (e2 < Hh - 2) { ly = base + 1 + e2; lx = base; }
-        else { ly = base + 1 + e2 - (Hh - 2); lx = G::H + G::TX + d - 1; }
+    last_stage = 4 - m;
+    if (e < 2 * TX) {                      // bands above and below the tile
+        const bool below = e >= TX;
+        ly = below ? H + TY - 1 + m : H - m;
+        lx = H + (below ? e - TX : e);
+    } else if (e < 2 * (TX + TY)) {        // bands left and right of the tile
+        const int e2 = e - 2 * TX;
+        const bool right = e2 >= TY;
+        ly = H + (right ? e2 - TY : e2);
+        lx = right ? H + TX - 1 + m : H - m;
+    } else {                               // corner q, offsets a (rows) and m - a (columns)
+        const int c = e - 2 * (TX + TY);
+        const int q = c / (m - 1), a = c - q * (m - 1) + 1;
+        ly = (q & 2) ? H + TY - 1 + a : H - a;
+        lx = (q & 1) ? H + TX - 1 + (m - a) : H - (m - a);
     }
     return true;
 }
 
-// Some cell of slot sl may take part in stage k (1..4): tile slots always, a ring slot if its
-// smallest ring d has d <= 4-k (which of its cells do is decided per thread).
+// Some cell of slot sl may take part in stage k (1..4): tile slots always, a halo slot if its
+// smallest shell m has m <= 4-k (which of its cells do is decided per thread).
 template <class G>
 __device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || G::min_depth(sl - G::TS) <= 4 - k; }
 
@@ -579,7 +591,8 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 // template argument) are measured choices: the 64x16/512 forward step at 8 waves per SIMD (64
 // VGPRs, two spilled dwords) runs 8% faster than at the 6 its 67 VGPRs allow; the 64x8/512 adjoint
 // step fits 6 waves without spilling.  Requests that force larger spills (64x16/512 adjoint at 6
-// waves: 190 us instead of 120 us per 16 x 512^2 launch) are not made.
+// waves: 190 us instead of 115 us per 16 x 512^2 launch) are not made; at 5 waves it would spill
+// ten dwords and still hold two workgroups per CU (its 45.5 KB of LDS allow three, which need 6).
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream)
 {

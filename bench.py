@@ -24,11 +24,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell of one launch of the fused kernels (forward 64x16/512, adjoint 64x8/256 tiles, the
-# benchmark's launch shapes), measured with rocprofv3
-# --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2 batch and scaled by the factors a 1 GiB copy kernel
-# gives those counters (x2.0 and x1.0); forward is the mean of forward-sweep and recompute launches
-HBM_BYTES_PER_CELL = {"adj": 74.1, "fwd": 34.8}
+# HBM bytes per cell of one launch of the fused kernels (forward and adjoint on 64x16/512 tiles, the
+# benchmark's launch shapes), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2
+# batch and scaled by the factors a 1 GiB copy kernel gives those counters (x2.0 and x1.0); forward is
+# the mean of forward-sweep and recompute launches
+HBM_BYTES_PER_CELL = {"adj": 75.5, "fwd": 32.8}
+# VALU lane-instructions per cell of one launch, same launch shapes and runs (SQ_INSTS_VALU x 64 / cells);
+# the issue peak is one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz (non-packed FP32)
+VALU_LANE_INSTR_PER_CELL = {"adj": 831.0, "fwd": 526.0}
+VALU_PEAK_G_WAVE_INSTR_S = 256 * 2.4
 
 
 def log(*a):
@@ -132,12 +136,20 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
         dom, dbytes, dms, dkey = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms, "fwd"
     achieved = dbytes / (dms * 1e-3) / 1e9
     eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
+
+    def valu(key, ms):
+        rate = VALU_LANE_INSTR_PER_CELL[key] / 64 * cells / (ms * 1e-3) / 1e9
+        return {"lane_instr_per_cell": VALU_LANE_INSTR_PER_CELL[key], "achieved": rate,
+                "peak": VALU_PEAK_G_WAVE_INSTR_S, "unit": "G wave-instr/s", "frac": rate / VALU_PEAK_G_WAVE_INSTR_S}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS, "traffic": HBM_BYTES_PER_CELL[dkey] * cells, "kernel": dom,
             "kernel_ms": dms, "bytes_per_launch": dbytes,
             "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
             "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},
-            "eval_GBs": eval_bytes * evals / seconds / 1e9}
+            "eval_GBs": eval_bytes * evals / seconds / 1e9,
+            # the binding limit of these kernels: VALU issue (instruction counts from rocprofv3 counters
+            # on the benchmark's launch shapes, timing from this run)
+            "valu": {"fwd": valu("fwd", fwd_ms), "adj": valu("adj", adj_ms)}}
 
 
 def run_c1(args):

@@ -100,7 +100,8 @@ def test_fp32_checkpoint_and_tile_invariance(crop):
     Et = _Et(nt, 2)
     S = G.S_matrix(crop.mask)
     out = []
-    cfgs = ((1, 1, 1), (7, 1, 1), (40, 1, 1), (13, 2, 1), (9, 3, 3), (40, 3, 1), (11, 4, 4), (40, 5, 5), (6, 4, 2))
+    cfgs = ((1, 1, 1), (7, 1, 1), (40, 1, 1), (13, 2, 1), (9, 3, 3), (40, 3, 1), (11, 4, 4), (40, 5, 5), (6, 4, 2),
+            (8, 6, 6), (10, 7, 7), (40, 7, 1))
     for ck, ft, at in cfgs:
         with fkadj.Instance(crop.ai, crop.nrows, crop.ncols, dt=dt, nt=nt, ghost_lr=crop.gh, alpha=meta["alpha"],
                             gamma=meta["gamma"], S=S, precision=fkadj.F32, ckpt=ck, fwd_tile=ft,

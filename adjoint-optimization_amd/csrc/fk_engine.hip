@@ -151,6 +151,7 @@ struct EvalCtx {
     char *d_ck = nullptr, *d_seg = nullptr, *d_ring = nullptr, *d_z = nullptr, *d_work = nullptr;
     char *d_kseg = nullptr, *d_kT = nullptr;          // FP32: stored slopes of the recompute sweep
     char *d_kfirst = nullptr;                         // FP32: first slope of each segment, mod 3
+    int seg_copies = 1;                               // FP32: segment state / slope buffers (1 or 2)
     FusedSlot slot_h{};
     struct FusedGroupDev *group = nullptr;            // this context alone (FP32)
     std::vector<double> Ef, Ea;                       // host tables of the last evaluation
@@ -442,15 +443,33 @@ struct Sampler {
 };
 
 // ------------------------------------------------------------------------ evaluation contexts
+// The FP32 backward pass overlaps the recompute of segment s-1 with the adjoint of segment s (two
+// streams, two segment buffers) when its group has fewer cells than this; larger groups fill the
+// GPU with one stream and need one segment buffer.
+constexpr int64_t kTwoStreamCells = (int64_t)2 << 20;
+
+// Segment state / slope buffers of an FP32 context: two only where an overlapped backward pass can
+// happen, i.e. for an instance's own context (it may join a group) smaller than the threshold.  A
+// server request's context and a context of an instance at or above the threshold take one.
+int seg_copies_of(int prec, int64_t cells, bool own_context)
+{
+    return prec == FKADJ_F32 && own_context && cells < kTwoStreamCells ? 2 : 1;
+}
+
+// States held by an evaluation context.  FP64: checkpoints, one segment, ring, adjoint pair, 6 stage
+// planes; FP32: checkpoints, c segments, ring, adjoint pair, c x K stored slopes, f(y_nt), 3 first
+// slopes.
+int64_t ctx_states(int prec, int64_t nck, int64_t K, int c)
+{
+    return prec == FKADJ_F64 ? nck + (K + 1) + 2 + 2 + 6 : nck + c * (K + 1) + 2 + 2 + c * K + 1 + 3;
+}
+
 // Device bytes of one evaluation context (tables, partials, checkpoints, segment states, ring,
-// adjoint ping-pong, FP64 stage scratch or FP32 stored slopes).
-int64_t ctx_bytes(const fkadj_instance *I)
+// adjoint ping-pong, FP64 stage scratch or FP32 stored slopes) with c segment buffers.
+int64_t ctx_bytes(const fkadj_instance *I, int c)
 {
     const int64_t st = (int64_t)I->st_bytes;
-    // FP64: checkpoints, one segment, ring, adjoint pair, 6 stage planes; FP32: checkpoints, two
-    // segments, ring, adjoint pair, 2K stored slopes, f(y_nt), 3 first slopes
-    int64_t states = I->prec == FKADJ_F64 ? I->nck + (I->K + 1) + 2 + 2 + 6
-                                          : I->nck + 2 * (I->K + 1) + 2 + 2 + 2 * I->K + 1 + 3;
+    const int64_t states = ctx_states(I->prec, I->nck, I->K, c);
     int64_t tables = 8 * (int64_t)(6 * I->nt + (int64_t)I->nt * I->nb_part + I->nb_plain + (I->nt + 1) +
                                    (I->n_num + 255) / 256 + 1);
     return states * st + tables;
@@ -475,6 +494,7 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
     auto *C = new (std::nothrow) EvalCtx();
     if (!C) return FKADJ_ERR_NOMEM;
     C->I = I;
+    C->seg_copies = seg_copies_of(I->prec, (int64_t)I->rows * I->pitch, self_group);
     const size_t st = I->st_bytes;
     auto alc = [&](char *&q, size_t bytes) -> bool {
         if (hipMalloc((void **)&q, bytes) != hipSuccess) { q = nullptr; return false; }
@@ -487,15 +507,17 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
     bool ok = ald(C->d_Ef, 3 * (size_t)nt) && ald(C->d_Ea, 3 * (size_t)nt) && ald(C->d_part, (size_t)nt * I->nb_part) &&
               ald(C->d_partT, I->nb_plain) && ald(C->d_zgh, nt + 1) && ald(C->d_L0part, objective_blocks(I->n_num)) &&
               ald(C->d_L0, 1) && alc(C->d_ck, st * I->nck) &&
-              alc(C->d_seg, st * (I->K + 1) * (I->prec == FKADJ_F32 ? 2 : 1)) &&
+              alc(C->d_seg, st * (I->K + 1) * C->seg_copies) &&
               alc(C->d_ring, st * 2) && alc(C->d_z, st * 2);
     if (ok && I->prec == FKADJ_F64) ok = alc(C->d_work, st * 6);
-    if (ok && I->prec == FKADJ_F32) ok = alc(C->d_kseg, st * 2 * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
+    if (ok && I->prec == FKADJ_F32)
+        ok = alc(C->d_kseg, st * C->seg_copies * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
     if (ok && I->prec == FKADJ_F32) {
         FusedSlot &S = C->slot_h;
         S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
         S.nt = I->nt; S.K = I->K; S.nck = I->nck;
         S.ntiles = I->nb_part;
+        S.seg_par = C->seg_copies - 1;
         S.code = I->d_code;
         S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
         S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT); S.kfirst = as<float>(C->d_kfirst);
@@ -511,7 +533,7 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
         ctx_free(C);
         return FKADJ_ERR_NOMEM;
     }
-    C->bytes = ctx_bytes(I);
+    C->bytes = ctx_bytes(I, C->seg_copies);
     *out = C;
     return FKADJ_OK;
 }
@@ -662,11 +684,16 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     // buffers, so the recompute of segment s-2 waits for the adjoint of segment s.  Launches of
     // fewer than 2M cells leave the GPU part idle and gain from the overlap (single 512^2 instance:
     // evaluation 325 -> 263 ms; 256^2: 238 -> 181 ms); a full GPU does not (16 x 512^2: 1% slower),
-    // so larger groups keep both sweeps on one stream (the events then order nothing new).
+    // so larger groups, and groups with a member of one segment buffer (only contexts of 2M cells
+    // or more have one), keep both sweeps on one stream (the events then order nothing new).
     const int nseg = (nt + K - 1) / K;
     int64_t cells = 0;
-    for (auto *C : g->members) cells += (int64_t)C->I->rows * C->I->pitch;
-    hipStream_t side = cells < ((int64_t)2 << 20) ? g->side : s;
+    bool two_buffers = true;
+    for (auto *C : g->members) {
+        cells += (int64_t)C->I->rows * C->I->pitch;
+        two_buffers = two_buffers && C->seg_copies == 2;
+    }
+    hipStream_t side = cells < kTwoStreamCells && two_buffers ? g->side : s;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt), p = sg & 1;
         if (sg + 2 <= nseg - 1) HIPCHK(hipStreamWaitEvent(s, g->ev_adj[p], 0));
@@ -796,7 +823,7 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t esz = p->precision == FKADJ_F32 ? 4 : 8;
     const int64_t st = 3 * n * esz;
     const int64_t nck = (p->nt + K - 1) / K + 1;
-    int64_t states = 1 + (p->precision == FKADJ_F64 ? nck + (K + 1) + 2 + 2 + 6 : nck + 2 * (K + 1) + 2 + 2 + 2 * K + 1 + 3);
+    const int64_t states = 1 + ctx_states(p->precision, nck, K, seg_copies_of(p->precision, n, true));
     const int acfg = p->adj_tile > 0 && p->adj_tile < NUM_TILE_CFG ? p->adj_tile : default_adj_cfg(p->nrows, pitch);
     int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : tiles_of(p->nrows, pitch, acfg);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
@@ -949,6 +976,10 @@ int fkadj_destroy(fkadj_instance *I)
 }
 
 int fkadj_n_active(const fkadj_instance *I) { return I ? I->n_num : FKADJ_ERR_ARG; }
+int64_t fkadj_request_bytes(const fkadj_instance *I)
+{
+    return I ? ctx_bytes(I, seg_copies_of(I->prec, (int64_t)I->rows * I->pitch, false)) : FKADJ_ERR_ARG;
+}
 int fkadj_n_controls(const fkadj_instance *I) { return I ? I->ncontrols : FKADJ_ERR_ARG; }
 
 int fkadj_set_state(fkadj_instance *I, const double *y)
@@ -1420,7 +1451,7 @@ int server_admit(fkadj_server *S)
         if (free_slot < 0) return FKADJ_OK;
         ServReq *r = S->queue[q];
         fkadj_instance *I = r->I;
-        const int64_t bytes = ctx_bytes(I);
+        const int64_t bytes = fkadj_request_bytes(I);
         if (bytes > S->budget) {   // can never fit: fail it instead of blocking the queue
             r->rc = FKADJ_ERR_NOMEM;
             r->state = RQ_FAILED;

@@ -80,12 +80,13 @@ struct FusedSlot {
     int rows, pitch, n;
     int nt, K, nck;             // steps, checkpoint interval, checkpoints
     int ntiles;                 // tiles of this instance (zgh partial row length)
+    int seg_par;                // 1: seg and kseg hold two buffers, chosen by segment parity; 0: one
     const uint16_t *code;
     float *ck;                  // nck states: ck[c] = state at step min(c*K, nt)
-    float *seg;                 // 2 x (K+1) states of recomputed segments, by segment parity (state 0 unused)
+    float *seg;                 // (seg_par+1) x (K+1) states of recomputed segments (state 0 unused)
     float *ring;                // 2 states for the forward sweep between checkpoints
     float *zb;                  // 2 adjoint states (ping-pong)
-    float *kseg;                // 2 x K slopes f(y_j) of recomputed steps (double-buffered by segment parity)
+    float *kseg;                // (seg_par+1) x K slopes f(y_j) of recomputed steps
     float *kfirst;              // 3 slopes: f(y_a) of the segment starting at a, by segment index mod 3
     float *kT;                  // slope f(y_nt) at the final time with the adjoint's field value
     const double *Ef, *Ea;      // stage field tables, 3 per step

@@ -267,15 +267,15 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
         yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
                                              : S.ring + (size_t)(nx & 1) * st;
     } else if (mode == MODE_REC) {
-        // segments alternate between two state buffers and two slope buffers, so the recompute of
+        // with two buffers (seg_par 1) segments alternate between them, so the recompute of
         // segment s-1 may run while the adjoint steps of segment s read segment s's buffers; the
         // first slope of each segment is also kept by segment index mod 3 for the adjoint step
         // that ends at it (the last one of the previous segment)
         const int j = step - seg_a, segi = seg_a / S.K;
-        float *segb = S.seg + (size_t)(segi & 1) * (S.K + 1) * st;
+        float *segb = S.seg + (size_t)(segi & S.seg_par) * (S.K + 1) * st;
         yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
         yout = segb + (size_t)(j + 1) * st;
-        kout = S.kseg + ((size_t)(segi & 1) * S.K + j) * st;
+        kout = S.kseg + ((size_t)(segi & S.seg_par) * S.K + j) * st;
         if (j == 0) kout2 = S.kfirst + (size_t)(segi % 3) * st;
     } else {
         yin = S.ck + (size_t)(S.nck - 1) * st;
@@ -397,8 +397,8 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     const size_t st = (size_t)3 * S.n;
     const int segi = seg_a / S.K;
     const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
-    const float *kbuf = S.kseg + (size_t)(segi & 1) * S.K * st;
-    const float *segb = S.seg + (size_t)(segi & 1) * (S.K + 1) * st;
+    const float *kbuf = S.kseg + (size_t)(segi & S.seg_par) * S.K * st;
+    const float *segb = S.seg + (size_t)(segi & S.seg_par) * (S.K + 1) * st;
     const auto *yA = gmem((i - 1 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 1 - seg_a) * st);
     const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st);
     const auto *kA = gmem(kbuf + (size_t)(i - 1 - seg_a) * st);

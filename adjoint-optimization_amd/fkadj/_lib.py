@@ -78,6 +78,8 @@ def lib():
     L.fkadj_destroy.argtypes = [vp]
     L.fkadj_footprint_bytes.argtypes = [C.POINTER(Problem)]
     L.fkadj_footprint_bytes.restype = C.c_int64
+    L.fkadj_request_bytes.argtypes = [vp]
+    L.fkadj_request_bytes.restype = C.c_int64
     L.fkadj_set_state.argtypes = [vp, dp]
     L.fkadj_get_state.argtypes = [vp, dp]
     L.fkadj_n_active.argtypes = [vp]

@@ -93,6 +93,8 @@ class Instance:
         self.h = h
         self.n_active = L.lib().fkadj_n_active(h)
         self.n_controls = L.lib().fkadj_n_controls(h)
+        # device bytes one server request on this instance holds while admitted
+        self.request_bytes = int(L.lib().fkadj_request_bytes(h))
 
     def close(self):
         """Frees the instance; raises FkadjError (ERR_BUSY) while a server request or a batch plan

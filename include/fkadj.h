@@ -122,6 +122,10 @@ int fkadj_destroy(fkadj_instance *inst);
 
 /* Device bytes held by an instance with this problem (for admission by memory footprint). */
 int64_t fkadj_footprint_bytes(const fkadj_problem *p);
+/* Device bytes one server request on this instance holds while it is admitted (its evaluation
+ * buffers), the figure the server checks against its budget; negative on a bad argument.  No
+ * reference counterpart. */
+int64_t fkadj_request_bytes(const fkadj_instance *inst);
 
 /* y_packed: 3*n_num values [u; v; w].  Stays resident until replaced. */
 int fkadj_set_state(fkadj_instance *inst, const double *y_packed);
@@ -184,8 +188,7 @@ int fkadj_last_timing(fkadj_timing *t);
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching gradient server on one GPU (one host thread drives it).  A request is
  * admitted when a slot is free and its evaluation buffers (checkpoints, segment states, slopes,
- * adjoint states, tables: the per-evaluation part of fkadj_footprint_bytes) fit the remaining
- * budget; the buffers are allocated at admission and freed when the request finishes.  Admitted
+ * adjoint states, tables: fkadj_request_bytes of its instance) fit the remaining budget; the buffers are allocated at admission and freed when the request finishes.  Admitted
  * requests advance together, one forward-kind and one adjoint launch per round over all slots,
  * each slot at its own step and phase; new requests join and finished ones leave at round
  * boundaries.  Each result equals the same request evaluated alone (fkadj_grad on the instance)

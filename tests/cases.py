@@ -55,7 +55,8 @@ def disc_case(H, W, n_holes, seed, rmin=1, rmax=3):
     rng = np.random.default_rng(seed + 1)
     I, J = np.mgrid[0:H, 0:W]
     front = 0.35 * W + 0.15 * W * np.sin(2 * np.pi * I / H + rng.uniform(0, 6))
-    u = 1.0 / (1.0 + np.exp((J - front) / 1.5))
+    with np.errstate(over="ignore"):   # far behind the front exp overflows to inf and u is 0
+        u = 1.0 / (1.0 + np.exp((J - front) / 1.5))
     v = 1.0 - 0.9 * u
     w = 0.95 - 0.3 * u
     return Case(mask, np.stack([u, v, w]))

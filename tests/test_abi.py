@@ -91,6 +91,20 @@ def test_footprint_grows_with_size():
     assert 0.9e9 < fkadj.footprint_bytes(p) < 1.4e9
 
 
+def test_footprint_one_segment_buffer_from_2M_cells():
+    """FP32 instances below 2M cells keep two segment buffers (their backward pass may overlap a
+    recompute with an adjoint), larger ones one: one row fewer than 2M cells costs K+1 states and K
+    slopes more (101 FP32 states of 1023 x 2048 cells = 2.54 GB at K = 50)."""
+    p = L.Problem()
+    p.nt, p.ckpt, p.precision = 1000, 50, L.F32
+    p.nrows, p.ncols = 1024, 2048
+    one = fkadj.footprint_bytes(p)
+    p.nrows = 1023
+    two = fkadj.footprint_bytes(p)
+    state = 3 * 4 * 1023 * 2048
+    assert 100 * state < two - one < 102 * state
+
+
 def test_disc_holes_have_no_isthmus_and_closed_form_ghost():
     m = G.disc_holes(96, 128, 25, 2, 6, seed=7)
     assert not G.isthmus_cells(m).any()

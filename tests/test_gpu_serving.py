@@ -69,7 +69,7 @@ def test_budget_admits_one_at_a_time_and_repeats_on_one_instance():
     _print_case("server_budget")
     pairs = _instances(SPECS[:3])
     alone = [inst.grad(A) for inst, A in pairs]
-    biggest = max(fkadj.footprint_bytes(inst.problem) for inst, _ in pairs)
+    biggest = max(inst.request_bytes for inst, _ in pairs)
     with fkadj.Server(budget_bytes=biggest, max_slots=4) as srv:
         ids = [srv.submit(inst, A) for inst, A in pairs]
         ids.append(srv.submit(*pairs[0]))            # the same instance again: runs after the first
@@ -86,9 +86,9 @@ def test_oversize_and_cancel_fail_only_themselves():
     _print_case("server_cancel")
     pairs = _instances(SPECS[:3])
     alone = [inst.grad(A) for inst, A in pairs]
-    small = min(fkadj.footprint_bytes(inst.problem) for inst, _ in pairs)
-    big = max(range(3), key=lambda k: fkadj.footprint_bytes(pairs[k][0].problem))
-    with fkadj.Server(budget_bytes=fkadj.footprint_bytes(pairs[big][0].problem), max_slots=4) as srv:
+    small = min(inst.request_bytes for inst, _ in pairs)
+    big = max(range(3), key=lambda k: pairs[k][0].request_bytes)
+    with fkadj.Server(budget_bytes=pairs[big][0].request_bytes, max_slots=4) as srv:
         ids = [srv.submit(inst, A) for inst, A in pairs]
         srv.run(5)
         victim = [k for k in range(3) if k != big][0]

@@ -1455,6 +1455,7 @@ int server_admit(fkadj_server *S)
         if (bytes > S->budget) {   // can never fit: fail it instead of blocking the queue
             r->rc = FKADJ_ERR_NOMEM;
             r->state = RQ_FAILED;
+            I->busy--;
             S->queue.erase(S->queue.begin() + q);
             continue;
         }
@@ -1473,6 +1474,7 @@ int server_admit(fkadj_server *S)
         if (rc != FKADJ_OK) {
             r->rc = rc;
             r->state = RQ_FAILED;
+            I->busy--;
             if (r->h_out) (void)hipHostFree(r->h_out);
             r->h_out = nullptr;
             if (r->C) (void)hipStreamSynchronize(S->stream);
@@ -1480,7 +1482,6 @@ int server_admit(fkadj_server *S)
             r->C = nullptr;
             continue;
         }
-        I->busy++;
         S->used += bytes;
         r->bytes = bytes;
         r->slot = free_slot;
@@ -1599,6 +1600,7 @@ int fkadj_server_destroy(fkadj_server *S)
     (void)server_reap(S, true);
     for (auto &kv : S->reqs) {
         ServReq *r = kv.second;
+        if (r->state == RQ_QUEUED) r->I->busy--;
         ctx_free(r->C);
         if (r->h_out) (void)hipHostFree(r->h_out);
         if (r->done) (void)hipEventDestroy(r->done);
@@ -1628,6 +1630,7 @@ int fkadj_server_submit(fkadj_server *S, fkadj_instance *I, const double *E, int
     r->E.assign(E, E + I->ncontrols);
     S->reqs[r->id] = r;
     S->queue.push_back(r);
+    I->busy++;   // released when the request finishes, fails or is cancelled
     *request_id = r->id;
     return FKADJ_OK;
 }
@@ -1643,6 +1646,7 @@ int fkadj_server_cancel(fkadj_server *S, int64_t id)
             if (S->queue[q] == r) { S->queue.erase(S->queue.begin() + q); break; }
         r->rc = FKADJ_ERR_CANCELLED;
         r->state = RQ_CANCELLED;
+        r->I->busy--;
     } else if (r->state == RQ_RUNNING) {
         r->cancel = true;
     }

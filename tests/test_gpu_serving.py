@@ -112,3 +112,27 @@ def test_oversize_and_cancel_fail_only_themselves():
         assert e.value.code == -3
     for inst, _ in pairs:
         inst.close()
+
+
+def test_queued_request_holds_its_instance():
+    """From submission on (queued, not yet admitted) the instance is busy: its state cannot be
+    replaced and it cannot be destroyed until the request is cancelled or finished."""
+    _print_case("server_queued_busy")
+    (inst, A), = _instances(SPECS[:1])
+    y0 = inst.get_state()
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=2) as srv:
+        rid = srv.submit(inst, A)
+        for call in (lambda: inst.set_state(y0), inst.close):
+            with pytest.raises(fkadj.FkadjError) as e:
+                call()
+            assert e.value.code == -8
+        srv.cancel(rid)
+        inst.set_state(y0)                          # free again once the queued request is cancelled
+        with pytest.raises(fkadj.FkadjError) as e:
+            srv.poll(rid)
+        assert e.value.code == -7
+        rid = srv.submit(inst, A)
+        while srv.pending():
+            srv.run(0)
+        srv.poll(rid)
+    inst.close()

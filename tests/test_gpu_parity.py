@@ -322,3 +322,35 @@ def test_fp32_c2_type_error_bound():
     eJ = abs(out[fkadj.F32][1] / out[fkadj.F64][1] - 1)
     print(f"c2-type fp32 vs fp64: grad {e:.3e}, J {eJ:.3e}")
     assert e < 1e-4 and eJ < 1e-4
+
+
+@pytest.mark.slow
+def test_fp32_c2_full_size_error_bound():
+    """The C2 configuration at its full size (512^2, 80 disc holes, S1-S2 spiral, T = 200 ms =
+    10 000 steps, 200 amplitudes, checkpoints every 50 steps): FP32 gradient and J within 1e-4 of
+    the FP64 path, and the instance evaluated inside a batch gives its own result bit for bit."""
+    _print_case("c2_full_fp32_vs_fp64", grid=512, T=200)
+    from fkadj import workloads as W
+    cfg = W.CONFIGS["C2"]
+    masks = [W.holes_mask(512, cfg["holes"], s) for s in (0, 1)]
+    states = W.spiral_states(masks, [0, 1])
+    As = [W.amplitudes(cfg["K"], s) for s in (0, 1)]
+    out = {}
+    for prec in (fkadj.F64, fkadj.F32):
+        inst = W.make_instance(masks[0], cfg["T"], cfg["K"], precision=prec, ckpt=cfg["ckpt"])
+        inst.set_state(states[0])
+        out[prec] = inst.grad(As[0])
+        inst.close()
+    e = relerr(out[fkadj.F32][0], out[fkadj.F64][0])
+    eJ = abs(out[fkadj.F32][1] / out[fkadj.F64][1] - 1)
+    print(f"c2 full size fp32 vs fp64: grad {e:.3e}, J {eJ:.3e}")
+    assert e < 1e-4 and eJ < 1e-4
+    insts = []
+    for mk, y in zip(masks, states):
+        inst = W.make_instance(mk, cfg["T"], cfg["K"], precision=fkadj.F32, ckpt=cfg["ckpt"])
+        inst.set_state(y)
+        insts.append(inst)
+    batched = fkadj.batch_grad(insts, As)
+    assert np.array_equal(batched[0][0], out[fkadj.F32][0]) and batched[0][1] == out[fkadj.F32][1]
+    for inst in insts:
+        inst.close()

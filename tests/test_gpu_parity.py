@@ -354,3 +354,46 @@ def test_fp32_c2_full_size_error_bound():
     assert np.array_equal(batched[0][0], out[fkadj.F32][0]) and batched[0][1] == out[fkadj.F32][1]
     for inst in insts:
         inst.close()
+
+
+def _sweep_case(seed):
+    """Seeded small problem with everything drawn at random: ragged shape (down to 4 x 4), hole
+    count, steps, pulse pieces, checkpoint interval and both tile configurations."""
+    rng = np.random.default_rng(1000 + seed)
+    H, W = int(rng.integers(4, 141)), int(rng.integers(4, 261))
+    holes = int(rng.integers(0, 1 + H * W // 500))
+    nt = int(rng.integers(6, 61))
+    divisors = [k for k in range(1, nt + 1) if nt % k == 0]
+    K = int(rng.choice(divisors))
+    ckpt = int(rng.integers(1, nt + 1))
+    ft = int(rng.choice([0, 1, 2, 3, 4, 5, 6, 7]))
+    at = int(rng.choice([0, 1, 3, 4, 5, 6, 7]))
+    return H, W, holes, nt, K, ckpt, ft, at
+
+
+@pytest.mark.parametrize("seed", range(16))
+def test_random_small_problems_match_oracle(seed):
+    """Random shapes, holes, horizons, pulse pieces, checkpoint intervals and tiles: FP64 within
+    1e-9 and FP32 within 1e-4 of the oracle (gradient and J).  The energy weight is 0, so the whole
+    gradient comes from the adjoint solve."""
+    H, W, holes, nt, K, ckpt, ft, at = _sweep_case(seed)
+    _print_case("random_small", seed=seed, shape=(H, W), holes=holes, nt=nt, n_amp=K, ckpt=ckpt, fwd_tile=ft,
+                adj_tile=at)
+    case = disc_case(H, W, holes, seed)
+    if case.n_act == 0:
+        pytest.skip("every cell is a hole")
+    import oracle as O
+    m = fkadj.reference_model(h=0.025)
+    om = O.reference_model(h=0.025)
+    dt, lam = 0.02, 0.0
+    A = 10.0 * np.random.default_rng(seed).uniform(0, 1, K)
+    S = G.mean_u2_S(case.n_act)
+    g_ref, Lc_ref, _, _ = oracle_grad_piecewise(case, dt, nt, A, (1, 0, 0), 2 * lam, S, om)
+    for prec, tol in ((fkadj.F64, 1e-9), (fkadj.F32, 1e-4)):
+        with fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                            alpha=2 * lam, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=prec, ckpt=ckpt,
+                            fwd_tile=ft, adj_tile=at) as inst:
+            inst.set_state(case.y0)
+            g, Lc, _, _ = inst.grad(A)
+        assert relerr(g, g_ref) < tol, (prec, relerr(g, g_ref))
+        assert abs(Lc - Lc_ref) < tol * abs(Lc_ref), (prec, Lc, Lc_ref)

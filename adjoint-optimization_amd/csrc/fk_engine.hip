@@ -1010,6 +1010,43 @@ static int pack_state(fkadj_instance *I, const char *planes, double *y)
     return FKADJ_OK;
 }
 
+// Forward solve from the resident state over nsteps with field E; snapshot ks (ks < max_snaps)
+// of the packed state after step (ks+1)*snap_every (F:794-813), the packed final state, and with
+// advance the final state as the new resident state.
+static int forward_run(fkadj_instance *I, const double *E, int nsteps, int snap_every, int max_snaps, double *snaps,
+                       double *y_final, int advance)
+{
+    if (!I || !E || nsteps < 0 || nsteps > I->nt || snap_every < 0 || max_snaps < 0) return FKADJ_ERR_ARG;
+    if (max_snaps > 0 && (!snaps || snap_every < 1)) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    if (I->busy) return FKADJ_ERR_BUSY;
+    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    CHK(ensure_default_ctx(I));
+    EvalCtx *C = I->ctx;
+    hipStream_t s = I->stream;
+    CHK(build_tables(I, C, E, s));
+    HIPCHK(hipMemcpyAsync(I->state(C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    int taken = 0;
+    for (int i = 0; i < nsteps; i++) {
+        if (I->prec == FKADJ_F32) {
+            group_forward(C->group, MODE_FWD, i, 0, s);
+        } else {
+            auto pc = plain_ctx<double>(I, C);
+            plain_forward_step(pc, as<double>(fwd_loc(I, C, i)), as<double>(fwd_loc(I, C, i + 1)), i);
+        }
+        HIPCHK(hipGetLastError());
+        if (snap_every > 0 && (i + 1) % snap_every == 0 && taken < max_snaps) {
+            CHK(pack_state(I, fwd_loc(I, C, i + 1), snaps + (size_t)taken * 3 * I->n_num));
+            taken++;
+        }
+    }
+    char *fin = fwd_loc(I, C, nsteps);
+    if (y_final) CHK(pack_state(I, fin, y_final));
+    if (advance) HIPCHK(hipMemcpyAsync(I->d_y0, fin, I->st_bytes, hipMemcpyDeviceToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return FKADJ_OK;
+}
+
 int fkadj_get_state(fkadj_instance *I, double *y)
 {
     if (!I || !y) return FKADJ_ERR_ARG;
@@ -1026,35 +1063,57 @@ int fkadj_grad(fkadj_instance *I, const double *E, double *grad, double *Lc, dou
 int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every, double *snaps, double *y_final,
                   int advance)
 {
-    if (!I || !E || nsteps < 0 || nsteps > I->nt) return FKADJ_ERR_ARG;
-    if (snap_every > 0 && !snaps && nsteps / snap_every > 1) return FKADJ_ERR_ARG;
-    if (!I->state_set) return FKADJ_ERR_STATE;
-    if (I->busy) return FKADJ_ERR_BUSY;
-    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
-    CHK(ensure_default_ctx(I));
-    EvalCtx *C = I->ctx;
-    hipStream_t s = I->stream;
-    CHK(build_tables(I, C, E, s));
-    HIPCHK(hipMemcpyAsync(I->state(C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, s));
-    const int nsnap = snap_every > 0 ? nsteps / snap_every - 1 : 0;   // FM:7-25: final state excluded
-    int taken = 0;
-    for (int i = 0; i < nsteps; i++) {
-        if (I->prec == FKADJ_F32) {
-            group_forward(C->group, MODE_FWD, i, 0, s);
-        } else {
-            auto pc = plain_ctx<double>(I, C);
-            plain_forward_step(pc, as<double>(fwd_loc(I, C, i)), as<double>(fwd_loc(I, C, i + 1)), i);
+    if (!I || nsteps < 0 || snap_every < 0) return FKADJ_ERR_ARG;
+    const int nsnap = snap_every > 0 ? fkadj_snapshot_count(nsteps, snap_every) : 0;
+    return forward_run(I, E, nsteps, snap_every, nsnap, snaps, y_final, advance);
+}
+
+int fkadj_snapshot_count(int nsteps, int snap_every)
+{
+    if (nsteps < 0 || snap_every < 1) return FKADJ_ERR_ARG;
+    // FM:7-25: nt/nstep - 1 when nstep divides nt (the final state is returned separately),
+    // floor(nt/nstep) otherwise; its relative integer test fails at nt = 0, which gives 0
+    return nsteps > 0 && nsteps % snap_every == 0 ? nsteps / snap_every - 1 : nsteps / snap_every;
+}
+
+int fkadj_forward_ref(double dt, const double *Et, int nt, const double *y_ini, const int32_t *active_index, int nrows,
+                      int ncols, int n_num, const double *gh_num, int nstep, int ns, double *ys, double *yf,
+                      char *param_str, size_t param_str_cap)
+{
+    if (!Et || !y_ini || !active_index || !gh_num || !yf || nstep < 1 || ns < 0 || nt < 1) return FKADJ_ERR_ARG;
+    if (ns > nt / nstep || (ns > 0 && !ys)) return FKADJ_ERR_ARG;
+    fkadj_problem p{};
+    p.nrows = nrows;
+    p.ncols = ncols;
+    p.active_index = active_index;
+    p.gh_num = gh_num;
+    fkadj_model_reference(&p.model);
+    p.dt = dt;
+    p.nt = nt;
+    p.pulse_mode = FKADJ_PULSE_KNOTS;
+    p.gamma[0] = 1.0;
+    p.precision = FKADJ_F64;
+    p.lazy_workspace = 1;
+    fkadj_instance *I = nullptr;
+    int rc = fkadj_create(&p, &I);
+    if (rc) return rc;
+    if (I->n_num != n_num) { fkadj_destroy(I); return FKADJ_ERR_GEOMETRY; }
+    const size_t n = (size_t)nrows * ncols, np = (size_t)3 * n_num;
+    std::vector<double> packed(np * std::max(ns, 1)), fin(np);
+    rc = fkadj_set_state(I, y_ini);
+    if (!rc) rc = forward_run(I, Et, nt, nstep, ns, packed.data(), fin.data(), 0);
+    fkadj_destroy(I);
+    if (rc) return rc;
+    // the reference's output layout: three full planes per state (row-major i*ncols + j), NaN at holes
+    auto unpack = [&](const double *src, double *dst) {
+        for (size_t k = 0; k < n; k++) {
+            const int a = active_index[k];
+            for (int v = 0; v < 3; v++) dst[v * n + k] = a > 0 ? src[(size_t)v * n_num + a - 1] : std::nan("");
         }
-        HIPCHK(hipGetLastError());
-        if (snap_every > 0 && (i + 1) % snap_every == 0 && taken < nsnap) {
-            CHK(pack_state(I, fwd_loc(I, C, i + 1), snaps + (size_t)taken * 3 * I->n_num));
-            taken++;
-        }
-    }
-    char *fin = fwd_loc(I, C, nsteps);
-    if (y_final) CHK(pack_state(I, fin, y_final));
-    if (advance) HIPCHK(hipMemcpyAsync(I->d_y0, fin, I->st_bytes, hipMemcpyDeviceToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    };
+    for (int ks = 0; ks < ns; ks++) unpack(packed.data() + (size_t)ks * np, ys + (size_t)ks * 3 * n);
+    unpack(fin.data(), yf);
+    if (param_str && param_str_cap) fkadj_param_string(&p.model, param_str, param_str_cap);
     return FKADJ_OK;
 }
 

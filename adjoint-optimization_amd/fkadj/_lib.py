@@ -86,6 +86,9 @@ def lib():
     L.fkadj_n_controls.argtypes = [vp]
     L.fkadj_grad.argtypes = [vp, dp, dp, dp, dp, dp]
     L.fkadj_forward.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
+    L.fkadj_snapshot_count.argtypes = [C.c_int, C.c_int]
+    L.fkadj_forward_ref.argtypes = [C.c_double, dp, C.c_int, dp, ip32, C.c_int, C.c_int, C.c_int, dp, C.c_int,
+                                    C.c_int, dp, dp, C.c_char_p, C.c_size_t]
     L.fkadj_stimulate.argtypes = [vp, C.POINTER(C.c_uint8), C.c_double]
     L.fkadj_batch_grad.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.POINTER(dp), dp, dp, dp]
     L.fkadj_batch_forward.argtypes = [C.c_int, C.POINTER(vp), C.POINTER(dp), C.c_int]

@@ -53,6 +53,38 @@ def fk_grad_Et_mex(Et, dt, nt, y0, active_index, nrows, ncols, ghost_lr, gamma, 
     return gradE, Lc.value, L0.value, LE.value, buf.value.decode()
 
 
+def snapshot_count(nt: int, nstep: int) -> int:
+    """Snapshots a forward run of nt steps keeps every nstep steps (the reference's compute_ns,
+    src/cuda/fk_forward_Et_param_mex.cpp:7-25): nt/nstep - 1 when nstep divides nt, else floor."""
+    n = L.lib().fkadj_snapshot_count(int(nt), int(nstep))
+    if n < 0:
+        check(n, "fkadj_snapshot_count")
+    return n
+
+
+def fk_forward_Et_mex(Et, dt, nt, y0, active_index, nrows, ncols, ghost_lr, nstep):
+    """Reference-shaped forward call (FP64 GPU path), src/cuda/fk_forward_Et_param_mex.cpp:40-121.
+    Returns (ys, yf, param_str): ys is (3*nrows*ncols, ns) with snapshot ks in column ks, yf is
+    (3*nrows*ncols,); each state is the u, v and w planes (cell k = i*ncols + j), NaN at holes."""
+    Et = L._f64(np.ravel(Et))
+    y0 = L._f64(np.ravel(y0))
+    ai = np.ascontiguousarray(np.ravel(active_index), dtype=np.int32)
+    gh = L._f64(np.ravel(ghost_lr))
+    nt, nstep = int(nt), int(nstep)
+    if Et.size != nt + 1:
+        raise ValueError("Et must have nt+1 samples")
+    ns = snapshot_count(nt, nstep)
+    n = int(nrows) * int(ncols)
+    ys = np.zeros((max(ns, 1), 3 * n))
+    yf = np.zeros(3 * n)
+    buf = C.create_string_buffer(2048)
+    rc = L.lib().fkadj_forward_ref(float(dt), L._d(Et), nt, L._d(y0), ai.ctypes.data_as(C.POINTER(C.c_int32)),
+                                   int(nrows), int(ncols), y0.size // 3, L._d(gh), nstep, ns, L._d(ys), L._d(yf),
+                                   buf, len(buf))
+    check(rc, "fkadj_forward_ref")
+    return ys[:ns].T.copy(), yf, buf.value.decode()
+
+
 class Instance:
     """A device-resident gradient problem (one grid, one objective, one resident state)."""
 
@@ -140,7 +172,7 @@ class Instance:
         """Forward solve; returns (snapshots (m, 3*n_active), final packed state)."""
         E = L._f64(np.ravel(E))
         nsteps = self.nt if nsteps is None else int(nsteps)
-        nsnap = max(nsteps // snap_every - 1, 0) if snap_every > 0 else 0
+        nsnap = snapshot_count(nsteps, snap_every) if snap_every > 0 else 0
         snaps = np.zeros((max(nsnap, 1), 3 * self.n_active))
         yf = np.zeros(3 * self.n_active)
         check(L.lib().fkadj_forward(self.h, L._d(E), nsteps, int(snap_every), L._d(snaps), L._d(yf),

@@ -18,6 +18,17 @@ GRAD_DESC_VARS = ("alpha", "ds", "ns", "final_time", "gamma", "nmod", "Et_seed_f
 _ASSIGN = re.compile(r"^\s*(\w+)\s*=(.*)$")
 
 
+def float_integer(r: float) -> int:
+    """r as an int when it is one to within 5e-16*r, 0 for 0, else ValueError
+    (src/cuda/float_integer.m:1-13)."""
+    if r == 0:
+        return 0
+    rn = round(r)
+    if abs(r - rn) < 5e-16 * abs(r):
+        return int(rn)
+    raise ValueError(f"{r!r} is not a float integer")
+
+
 def _strip_comment(line: str) -> str:
     """Drops a % comment that is not inside a quoted string."""
     quote = None
@@ -111,10 +122,7 @@ class GradDescentConfig:
     @property
     def nt(self) -> int:
         """Steps of one solve; T/dt must be an integer (float_integer, simple_grad_desc.m:53)."""
-        n = self.final_time / self.dt
-        if abs(n - round(n)) > 1e-9 * max(1.0, abs(n)):
-            raise ValueError(f"final_time/dt = {n} is not an integer")
-        return int(round(n))
+        return float_integer(self.final_time / self.dt)
 
     def as_dict(self) -> dict:
         return asdict(self)

@@ -10,7 +10,9 @@
  *   fk_grad_Et_comp          src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1572  -> fkadj_grad_ref
  *   mexFunction (gradient)   src/cuda/fk_grad_Et_param_mex.cpp:11-92           -> a MEX gateway over
  *                                                                                 fkadj_grad_ref
- *   fk_forward_Et_comp       src/cuda/fk_forward_Et_mono_PARAMETERS.cu:646     -> fkadj_forward
+ *   fk_forward_Et_comp       src/cuda/fk_forward_Et_mono_PARAMETERS.cu:646     -> fkadj_forward_ref
+ *                                                                                 (fkadj_forward on a handle)
+ *   compute_ns               src/cuda/fk_forward_Et_param_mex.cpp:7-25         -> fkadj_snapshot_count
  *   compile-time constants   src/cuda/fk_parameters_00_05_alpha_0p8_dx_0p035.h:2-24 -> fkadj_model
  *   parameters_string        src/cuda/fk_parameters_string_00_05_alpha_0p8_dx_0p035.cpp:1-32
  *                                                                              -> fkadj_param_string
@@ -139,11 +141,24 @@ int fkadj_grad(fkadj_instance *inst, const double *E, double *grad, double *Lcos
 
 /* Forward solve from the resident state over nsteps (<= nt) with field E (same layout as for
  * fkadj_grad).  With advance != 0 the final state replaces the resident state.
- * snap_every > 0: snapshots of the packed state every snap_every steps (excluding step 0 and,
- * as in src/cuda/fk_forward_Et_param_mex.cpp:7-25, excluding the final state) go to snaps,
- * (nsteps/snap_every - 1) * 3*n_num values.  y_final (may be NULL): packed final state. */
+ * snap_every > 0: snapshots of the packed state after steps snap_every, 2*snap_every, ...
+ * (fkadj_snapshot_count(nsteps, snap_every) of them; the final state is not among them) go to
+ * snaps, 3*n_num values each.  y_final (may be NULL): packed final state. */
 int fkadj_forward(fkadj_instance *inst, const double *E, int nsteps, int snap_every, double *snaps,
                   double *y_final, int advance);
+/* Number of snapshots a forward run of nsteps keeps every snap_every steps, as compute_ns
+ * (src/cuda/fk_forward_Et_param_mex.cpp:7-25): nsteps/snap_every - 1 when snap_every divides
+ * nsteps (the final state is returned separately), floor(nsteps/snap_every) otherwise. */
+int fkadj_snapshot_count(int nsteps, int snap_every);
+/* Drop-in for fk_forward_Et_comp (src/cuda/fk_forward_Et_mono_PARAMETERS.cu:646-849; gateway
+ * src/cuda/fk_forward_Et_param_mex.cpp:40-121), FP64 GPU path, knot-sampled field with the
+ * not-a-knot spline.  Same arguments and output layout: ys holds ns states after steps nstep,
+ * 2*nstep, ... (ns <= nt/nstep, normally fkadj_snapshot_count(nt, nstep)) and yf the final state,
+ * each three full planes of nrows*ncols values (u, v, w; cell k = i*ncols + j) with NaN at holes.
+ * Differences: an error code instead of exit(), the provenance string in a caller buffer. */
+int fkadj_forward_ref(double dt, const double *Et, int nt, const double *y_ini, const int32_t *active_index,
+                      int nrows, int ncols, int n_num, const double *gh_num, int nstep, int ns, double *ys,
+                      double *yf, char *param_str, size_t param_str_cap);
 
 /* Stimulus helper for building initial states: u = value on the cells where mask[k] != 0. */
 int fkadj_stimulate(fkadj_instance *inst, const uint8_t *cell_mask, double value);

@@ -8,7 +8,9 @@
  *      wrong sign, factor or time index fails by orders of magnitude);
  *   2. FP32 against FP64 (relative 1e-4);
  *   3. a batch of two FP32 instances, a batch plan (HIP graph) and a server, each equal to the
- *      instances' own results bit for bit.
+ *      instances' own results bit for bit;
+ *   4. the forward drop-in (fkadj_forward_ref): floor(40/15) = 2 snapshots in the reference's
+ *      full-grid layout with NaN at the hole, equal to a forward run on an FP64 instance.
  * Prints "c abi ok" and exits 0 on success; prints the failed check and exits 1 otherwise.
  */
 #include <math.h>
@@ -181,6 +183,41 @@ int main(void)
     CALL(fkadj_destroy(f32b));
     CALL(fkadj_destroy(f32a));
     CALL(fkadj_destroy(f64));
+
+    /* 4. forward drop-in against a forward run on a handle */
+    {
+        static double ys[2 * 3 * NR * NC], yf[3 * NR * NC], snaps[2 * 3 * NR * NC], fin[3 * NR * NC];
+        static double Et[NT + 1], gh[NR * NC];
+        const int nstep = 15, ns = fkadj_snapshot_count(NT, nstep);
+        char pstr[2048];
+        int i, v, s2, bad = 0;
+        CHECK(ns == 2, "snapshot count %d", ns);
+        for (i = 0; i <= NT; i++) Et[i] = 1.5 + sin(0.15 * i);
+        for (i = 0; i < n_num; i++) gh[i] = (double)(i % 3 - 1);   /* ghost_lr in {-1, 0, 1} */
+        make_problem(&p, FKADJ_F64, 0.4);
+        p.pulse_mode = FKADJ_PULSE_KNOTS;
+        p.n_amp = 0;
+        p.model.h = 0.035;              /* fkadj_forward_ref runs the reference's compile-time model */
+        p.gh_num = gh;
+        CALL(fkadj_forward_ref(0.02, Et, NT, y0v, active_index, NR, NC, n_num, gh, nstep, ns, ys, yf, pstr,
+                               sizeof pstr));
+        CALL(fkadj_create(&p, &f64));
+        CALL(fkadj_set_state(f64, y0v));
+        CALL(fkadj_forward(f64, Et, NT, nstep, snaps, fin, 0));
+        CALL(fkadj_destroy(f64));
+        for (s2 = 0; s2 <= ns; s2++) {
+            const double *full = s2 < ns ? ys + (size_t)s2 * 3 * NR * NC : yf;
+            const double *packed = s2 < ns ? snaps + (size_t)s2 * 3 * n_num : fin;
+            for (v = 0; v < 3; v++)
+                for (i = 0; i < NR * NC; i++) {
+                    const double x = full[(size_t)v * NR * NC + i];
+                    const int a = active_index[i];
+                    if (a == 0 ? !isnan(x) : x != packed[(size_t)v * n_num + a - 1]) bad++;
+                }
+        }
+        CHECK(bad == 0, "forward drop-in differs from the handle run at %d values", bad);
+        CHECK(strstr(pstr, "tau_w_plus") != NULL, "provenance string");
+    }
     if (fails) return 1;
     printf("c abi ok: adjoint vs central differences max rel %.2e; dJ/dA = %.6g %.6g %.6g %.6g\n", worst, g64[0],
            g64[1], g64[2], g64[3]);

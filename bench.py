@@ -177,12 +177,18 @@ def run_c1(args):
         res[name] = (out, (time.perf_counter() - ts) / args.steps)
         inst.close()
         log(f"[c1] {name}: {1e3 * res[name][1]:.1f} ms/eval")
-    g_cpu, Lc_cpu, sec = oracle_piecewise(m, y0, A, nt)
-    log(f"[c1] FP64 oracle on the host: {sec:.1f} s/eval")
-    g64, g32 = res["plain_f64"][0][0], res["fused_f32"][0][0]
+    runs = [oracle_piecewise(m, y0, A, nt) for _ in range(3)]   # median of 3 on all host threads
+    g_cpu, Lc_cpu, _ = runs[0]
+    sec = float(np.median([r[2] for r in runs]))
+    log(f"[c1] FP64 oracle on the host: {sec:.1f} s/eval (median of 3)")
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     threads = O.max_threads()
+    O.set_threads(1)
+    sec1 = oracle_piecewise(m, y0, A, nt)[2]
+    O.set_threads(threads)
+    log(f"[c1] FP64 oracle on one thread: {sec1:.1f} s/eval")
+    g64, g32 = res["plain_f64"][0][0], res["fused_f32"][0][0]
     out = {"metric": "gradient evals/s, C1 128^2 FP64 (plain GPU path; CPU oracle beside)",
            "value": 1.0 / res["plain_f64"][1], "unit": "evals/s", "n_gpus": 1, "steps": args.steps,
            "warmup": args.warmup, "ms_per_step": 1e3 * res["plain_f64"][1], "higher_is_better": True,
@@ -195,8 +201,10 @@ def run_c1(args):
            "gpu_f64_vs_cpu_oracle": {"grad": float(np.linalg.norm(g64 - g_cpu) / np.linalg.norm(g_cpu)),
                                      "J": float(abs(res["plain_f64"][0][1] / Lc_cpu - 1))},
            "cpu_baseline": {"value": 1.0 / sec, "unit": "evals/s", "cores": threads, "kind": "port",
-                            "sample": "FP64 oracle, the full C1 evaluation (forward, recompute and adjoint sweeps)",
-                            "seconds_per_eval": sec}}
+                            "sample": "FP64 oracle, the full C1 evaluation (forward, recompute and adjoint sweeps), "
+                                      "median of 3",
+                            "seconds_per_eval": sec, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
+                            "single_thread": {"seconds_per_eval": sec1, "sample": "one full C1 evaluation"}}}
     print(json.dumps(out), flush=True)
 
 

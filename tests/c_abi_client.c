@@ -3,9 +3,9 @@
  * Checks, on a small synthetic problem (48 x 80 cells, one 3x3 hole, piecewise pulse of 4
  * amplitudes, 40 steps):
  *   1. the FP64 gradient dJ/dA_k against central finite differences of J with alpha = 0, so the
- *      whole gradient comes from the adjoint solve (relative 1e-3; measured 4e-4: the backward
- *      RK4 with Hermite midpoints is a consistent approximation of the discrete derivative, and a
- *      wrong sign, factor or time index fails by orders of magnitude);
+ *      whole gradient comes from the adjoint solve (relative 1e-3; measured 4e-4, the method's own
+ *      second-order mismatch: the functional gradient at the knots integrated over each piece by the
+ *      trapezoid rule; a wrong sign, factor or time index fails by orders of magnitude);
  *   2. FP32 against FP64 (relative 1e-4);
  *   3. a batch of two FP32 instances, a batch plan (HIP graph) and a server, each equal to the
  *      instances' own results bit for bit;

@@ -9,6 +9,7 @@ What pins what:
   * jacT against central finite differences of F_ion
   * spline against scipy's independent not-a-knot CubicSpline and exactness on cubics
   * RK4 against its 4th-order convergence on a small grid
+  * the adjoint gradient against central differences of J in the pulse amplitudes
 """
 import numpy as np
 import pytest
@@ -163,3 +164,33 @@ def test_golden_iteration50_gradient(golden):
     G50 = d["gradE_50"]
     assert np.linalg.norm(g - G50) / np.linalg.norm(G50) < 1e-10
     assert abs(Lc - meta["iterations"]["50"]["Lcost"]) / meta["iterations"]["50"]["Lcost"] < 1e-12
+
+
+def test_gradient_matches_finite_differences():
+    """Independent of the golden data: dJ/dA_k from the oracle's adjoint against central differences
+    of J in the amplitudes (piecewise field, energy weight 0, so the whole gradient comes from the
+    adjoint solve).  The mismatch is the method's own (the functional gradient at the knots,
+    integrated over each piece by the trapezoid rule): second order in dt, so halving dt divides it
+    by four; a wrong sign, factor or time index would not converge at all."""
+    from cases import disc_case, oracle_grad_piecewise
+    case = disc_case(24, 40, 2, 5)
+    om = O.reference_model(h=0.025)
+    S = G.mean_u2_S(case.n_act)
+    A = np.array([3.0, 7.5, 1.0, 5.0])
+
+    def worst_mismatch(dt, nt):
+        g = oracle_grad_piecewise(case, dt, nt, A, (1, 0, 0), 0.0, S, om)[0]
+        worst = 0.0
+        for k in range(len(A)):
+            h = 1e-3 * (1 + abs(A[k]))
+            Ap, Am = A.copy(), A.copy()
+            Ap[k] += h
+            Am[k] -= h
+            Jp = oracle_grad_piecewise(case, dt, nt, Ap, (1, 0, 0), 0.0, S, om)[1]
+            Jm = oracle_grad_piecewise(case, dt, nt, Am, (1, 0, 0), 0.0, S, om)[1]
+            worst = max(worst, abs((Jp - Jm) / (2 * h) - g[k]) / abs(g[k]))
+        return worst
+
+    e1, e2 = worst_mismatch(0.02, 40), worst_mismatch(0.01, 80)
+    print(f"oracle adjoint vs central differences: max rel {e1:.2e} (dt 0.02), {e2:.2e} (dt 0.01)")
+    assert e1 < 1e-3 and 3.5 < e1 / e2 < 4.5, (e1, e2)

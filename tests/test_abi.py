@@ -151,3 +151,28 @@ def test_tile_table_lookup(tmp_path, monkeypatch):
     assert autotune.lookup(2048) == (3, 1)
     monkeypatch.setattr(autotune, "load_table", lambda path=None: {})
     assert autotune.lookup(512) == (0, 0)
+
+
+def test_forward_ref_argument_errors_before_device_work():
+    """fkadj_forward_ref rejects a snapshot interval below 1, more snapshots than nt/nstep and
+    missing arrays with FKADJ_ERR_ARG before it touches the device."""
+    import ctypes as C
+    lib = fkadj.lib()
+    nr = nc = 8
+    ai = np.arange(1, nr * nc + 1, dtype=np.int32)
+    y0 = np.zeros(3 * nr * nc)
+    gh = np.zeros(nr * nc)
+    Et = np.zeros(11)
+    ys = np.zeros(3 * nr * nc * 4)
+    yf = np.zeros(3 * nr * nc)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    aip = ai.ctypes.data_as(C.POINTER(C.c_int32))
+
+    def call(nstep, ns, ys_ptr=dp(ys), yf_ptr=dp(yf)):
+        return lib.fkadj_forward_ref(0.1, dp(Et), 10, dp(y0), aip, nr, nc, nr * nc, dp(gh), nstep, ns, ys_ptr,
+                                     yf_ptr, None, 0)
+
+    assert call(0, 0) == L.ERR_ARG
+    assert call(3, 4) == L.ERR_ARG            # floor(10/3) = 3 snapshots at most
+    assert call(3, 1, ys_ptr=None) == L.ERR_ARG
+    assert call(3, 1, yf_ptr=None) == L.ERR_ARG

@@ -169,6 +169,9 @@ struct FusedGroupDev {
     hipEvent_t ev_rec[2] = {}, ev_adj[2] = {};
     // one launch map per kernel configuration (index = tile config id); slot k = member k
     LaunchMap fwd[NUM_TILE_CFG], adj[NUM_TILE_CFG];
+    // forward sweep two steps per launch (64x8 tiles, all members) instead of one
+    bool depth2 = false;
+    LaunchMap fwd2;
 };
 
 namespace {
@@ -447,6 +450,8 @@ struct Sampler {
 // streams, two segment buffers) when its group has fewer cells than this; larger groups fill the
 // GPU with one stream and need one segment buffer.
 constexpr int64_t kTwoStreamCells = (int64_t)2 << 20;
+// Forward sweeps of groups below this many cells take two steps per launch.
+constexpr int64_t kDepth2Cells = (int64_t)1 << 20;
 
 // Segment state / slope buffers of an FP32 context: two only where an overlapped backward pass can
 // happen, i.e. for an instance's own context (it may join a group) smaller than the threshold.  A
@@ -631,12 +636,20 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
     }
     int64_t cells = 0;
     for (auto *C : members) cells += (int64_t)C->I->rows * C->I->pitch;
+    launch_map_clear(g->fwd2);
+    // Two forward steps per launch where one launch costs more than its arithmetic: launches of
+    // fewer than 1M cells (those the built-in rule gives 64x8 tiles), members on the built-in
+    // forward tile, and a step count and checkpoint interval that pair up.
+    const fkadj_instance *I0 = members[0]->I;
+    g->depth2 = cells < kDepth2Cells && I0->nt % 2 == 0 && I0->K % 2 == 0;
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k]->I;
         slots.push_back(members[k]->slot_h);
         const int fc = fwd_cfg_of(I, cells);
         launch_map_add(g->fwd[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
         launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
+        launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
+        g->depth2 = g->depth2 && I->fwd_cfg == 0;
     }
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
@@ -663,6 +676,12 @@ void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStr
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
 }
 
+void group_forward2(const FusedGroupDev *g, int step, hipStream_t s)
+{
+    const Cursors c = uniform_cursors((int)g->members.size(), MODE_FWD, step, 0);
+    fused_forward2_launch(g->d_slots, g->fwd2, c, s);
+}
+
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
@@ -675,7 +694,10 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     const int nt = I0->nt, K = I0->K;
     for (auto *C : g->members)
         HIPCHK(hipMemcpyAsync(C->I->state(C->d_ck, 0), C->I->d_y0, C->I->st_bytes, hipMemcpyDeviceToDevice, s));
-    for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
+    if (g->depth2)
+        for (int i = 0; i < nt; i += 2) smp.fwd_launch(s, [&] { group_forward2(g, i, s); });
+    else
+        for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
     HIPCHK(hipGetLastError());
     for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step

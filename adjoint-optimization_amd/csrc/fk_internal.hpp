@@ -140,6 +140,7 @@ inline int tiles_of(int rows, int pitch, int cfg) { return (pitch / FUSED_TX) * 
 // tile configuration.  Tiles whose slot cursor is not of the kernel's kind return at once.
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream);
+void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream);
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream);
 

@@ -1,6 +1,7 @@
 // fk_kernels_fused.hip -- FP32 fused time-step kernels over the tiles of one or many instances.
 //
-// One launch advances every listed tile by one full time step:
+// One launch advances every listed tile by one full time step (the forward sweep of small launches by
+// two, see k_fused_fwd's D):
 //   forward  : the four RK4 stages of G:460-563 (G = src/cuda/fk_grad_Et_mono_PARAMETERS.cu)
 //   adjoint  : Hermite midpoint (G:1244-1301), the four adjoint RK4 stages (G:1349-1427) and the
 //              z.gh partial (G:1457-1483)
@@ -26,29 +27,35 @@
 
 namespace fk {
 
-template <int TY, int NT_>
+// A 64 x TY tile advanced by D time steps per launch: 4D RK stages, a halo of 4D cells.
+template <int TY, int NT_, int D = 1>
 struct Tile {
-    static constexpr int TX = FUSED_TX, H = FUSED_HALO, NT = NT_;
+    static constexpr int TX = FUSED_TX, H = FUSED_HALO * D, NT = NT_;
+    static constexpr int LAST = 4 * D;               // stages of a launch; the tile's cells take part in all
+    static constexpr int NSH = LAST - 1;             // halo shells computed (shell m: stages 1..LAST-m)
     static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
     static constexpr int RXP = RX + 1;               // odd LDS row pitch: column walks hit distinct banks
     static constexpr int TS = TX * TY / NT;          // tile cells per thread
     static_assert(TX * TY % NT == 0, "tile must split evenly");
     // cells at Manhattan distance m from the tile: 2 TX + 2 TY in the bands, m - 1 per corner
     __host__ __device__ static constexpr int shell_count(int m) { return 2 * (TX + TY) + 4 * (m - 1); }
-    static constexpr int RC = shell_count(1) + shell_count(2) + shell_count(3);   // halo cells computed
-    static constexpr int NRS = (RC + NT - 1) / NT;                                 // halo slots per thread
+    static constexpr int RC = NSH * 2 * (TX + TY) + 2 * NSH * (NSH - 1);   // halo cells computed
+    static constexpr int NRS = (RC + NT - 1) / NT;                         // halo slots per thread
     static constexpr int NSLOT = TS + NRS;
-    // shell of the r-th halo cell in the shell-1, shell-2, shell-3 sequence
+    // shell of the r-th halo cell in the shell-1, shell-2, ... sequence
     __host__ __device__ static constexpr int depth_of(int r)
     {
-        return r < shell_count(1) ? 1 : (r < shell_count(1) + shell_count(2) ? 2 : 3);
+        int m = 1;
+        while (m < NSH && r >= shell_count(m)) r -= shell_count(m++);
+        return m;
     }
     // halo slot j holds sequence numbers NT*j .. NT*(j+1)-1: its smallest shell
     __host__ __device__ static constexpr int min_depth(int j) { return depth_of(NT * j); }
 };
 
 // Cell of slot `sl` of thread `tid` (LDS coordinates) and the last RK stage it takes part in
-// (4 for tile cells, 4-m for a shell-m cell); returns false if the thread owns no cell in that slot.
+// (LAST for tile cells, LAST-m for a shell-m cell); returns false if the thread owns no cell in
+// that slot.
 template <class G>
 __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_stage)
 {
@@ -57,7 +64,7 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_st
         int lin = tid + G::NT * sl;
         ly = H + lin / TX;
         lx = H + lin % TX;
-        last_stage = 4;
+        last_stage = G::LAST;
         return true;
     }
     const int j = sl - G::TS;
@@ -65,15 +72,13 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_st
     last_stage = 0;
     if (e >= G::RC) return false;
     int m = 1;
-    if (e >= G::shell_count(1)) {
-        e -= G::shell_count(1);
-        m = 2;
-        if (e >= G::shell_count(2)) {
-            e -= G::shell_count(2);
-            m = 3;
+#pragma unroll
+    for (int mm = 1; mm < G::NSH; mm++)
+        if (m == mm && e >= G::shell_count(mm)) {
+            e -= G::shell_count(mm);
+            m = mm + 1;
         }
-    }
-    last_stage = 4 - m;
+    last_stage = G::LAST - m;
     if (e < 2 * TX) {                      // bands above and below the tile
         const bool below = e >= TX;
         ly = below ? H + TY - 1 + m : H - m;
@@ -92,10 +97,13 @@ __device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_st
     return true;
 }
 
-// Some cell of slot sl may take part in stage k (1..4): tile slots always, a halo slot if its
-// smallest shell m has m <= 4-k (which of its cells do is decided per thread).
+// Some cell of slot sl may take part in stage k (1..LAST): tile slots always, a halo slot if its
+// smallest shell m has m <= LAST-k (which of its cells do is decided per thread).
 template <class G>
-__device__ constexpr bool slot_in_stage(int sl, int k) { return sl < G::TS || G::min_depth(sl - G::TS) <= 4 - k; }
+__device__ constexpr bool slot_in_stage(int sl, int k)
+{
+    return sl < G::TS || G::min_depth(sl - G::TS) <= G::LAST - k;
+}
 
 // Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0 in
 // LDS, so the mirror Laplacian (a missing neighbour takes the opposite one's value) is
@@ -198,21 +206,22 @@ __device__ inline TileRef launch_tile(const FusedSlot *__restrict__ slots, const
     return t;
 }
 
-// Region (tile + 4-cell halo) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
-// first 64 cells of a row (coalesced), lanes 0..7 the last 8 of 8 rows.  Cells outside the grid
-// read as 0 with code 0 (inactive).  All loads are issued before the first LDS store (their
-// addresses are clamped into the grid, so every load is unconditional and in bounds).
+// Region (tile + halo of H cells) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
+// first 64 cells of a row (coalesced), the last 2H cells of 64/(2H) rows per pass.  Cells outside
+// the grid read as 0 with code 0 (inactive).  All loads are issued before the first LDS store
+// (their addresses are clamped into the grid, so every load is unconditional and in bounds).
 template <class G>
 __device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
                                    float (*dst)[G::RY][G::RXP], uint16_t (*sc)[G::RXP])
 {
     constexpr int RY = G::RY, NW = G::NT / 64;
+    constexpr int TW = G::RX - 64, RPW = 64 / TW;       // tail width, tail rows per wavefront pass
+    static_assert(TW * RPW == 64, "the tail width must divide a wavefront");
     constexpr int NR = (RY + NW - 1) / NW;              // rows per wavefront (first 64 cells)
-    constexpr int NTL = (RY + 8 * NW - 1) / (8 * NW);   // tail passes (last 8 cells, 8 rows each)
+    constexpr int NTL = (RY + RPW * NW - 1) / (RPW * NW);   // tail passes
     constexpr int NE = NR + NTL;
-    static_assert(G::RX - 64 == 8, "tail loads assume an 8-cell halo");
     const int lane = tid & 63, wave = tid >> 6;
-    const int gx0 = x0 - FUSED_HALO, gy0 = y0 - FUSED_HALO;
+    const int gx0 = x0 - G::H, gy0 = y0 - G::H;
     const auto *g = gmem(src);
     const auto *gc = gmem(S.code);
     int lyv[NE], lxv[NE];
@@ -222,7 +231,7 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 #pragma unroll
     for (int e = 0; e < NE; e++) {
         if (e < NR) { lyv[e] = wave + e * NW; lxv[e] = lane; }
-        else { lyv[e] = (e - NR) * 8 * NW + wave * 8 + (lane >> 3); lxv[e] = 64 + (lane & 7); }
+        else { lyv[e] = (e - NR) * RPW * NW + wave * RPW + lane / TW; lxv[e] = 64 + lane % TW; }
         const int gy = gy0 + lyv[e], gx = gx0 + lxv[e];
         in[e] = lyv[e] < RY && (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
         const int k = in[e] ? gy * S.pitch + gx : 0;
@@ -240,18 +249,21 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 }
 
 // ------------------------------------------------------------------------------ forward
-// WPE: the occupancy asked of the register allocator (waves per SIMD; 1 = no request).
-template <int TY, int NT, int WPE>
+// WPE: the occupancy asked of the register allocator (waves per SIMD; 1 = no request).  D: time
+// steps per launch (D > 1 only for the forward sweep, MODE_FWD): the step-(st+1) state of every cell
+// within distance 4(D-1-st) of the tile is formed in the launch, so D steps read and write HBM once.
+template <int TY, int NT, int WPE, int D = 1>
 __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
-    using G = Tile<TY, NT>;
+    using G = Tile<TY, NT, D>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT;
+    constexpr int NA = D == 1 ? G::TS : NS;   // slots holding RK increments (cells that finish a step)
     __shared__ float sY[3][RY][RX];
     __shared__ uint16_t sC[RY][RX];
 
     const TileRef t = launch_tile<TY>(slots, map);
     const SlotCursor sc = cur.c[t.slot];
-    if (sc.mode != MODE_FWD && sc.mode != MODE_REC && sc.mode != MODE_RHS) return;   // uniform per block
+    if (sc.mode != MODE_FWD && (D > 1 || (sc.mode != MODE_REC && sc.mode != MODE_RHS))) return;   // uniform per block
     const int mode = sc.mode, step = sc.step, seg_a = sc.seg_a;
     const FusedSlot &S = slots[t.slot];
     const int tid = threadIdx.x;
@@ -262,10 +274,11 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
     float *kout2 = nullptr;     // second copy of the first slope of a segment (MODE_REC, j = 0)
     float E1, E23, E4;
     if (mode == MODE_FWD) {
-        yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
-        int nx = step + 1;
+        // between checkpoints the sweep alternates between the two ring states, launch by launch
+        yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)((step / D) & 1) * st;
+        int nx = step + D;
         yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
-                                             : S.ring + (size_t)(nx & 1) * st;
+                                             : S.ring + (size_t)((nx / D) & 1) * st;
     } else if (mode == MODE_REC) {
         // with two buffers (seg_par 1) segments alternate between them, so the recompute of
         // segment s-1 may run while the adjoint steps of segment s read segment s's buffers; the
@@ -297,7 +310,7 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
     bool own[NS];
     uint32_t cc[NS];
     LapW lw[NS];
-    float y[NS][3], acc[G::TS][3], f[NS][3];
+    float y[NS][3], acc[NA][3], f[NS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
         own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl], ls[sl]);
@@ -309,47 +322,71 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
     }
 
 #pragma unroll
-    for (int k = 1; k <= 4; k++) {
-        const float E = (k == 1) ? E1 : (k == 4 ? E4 : E23);
-#pragma unroll
-        for (int sl = 0; sl < NS; sl++) {
-            if (!slot_in_stage<G>(sl, k)) continue;
-            f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
-            if (k <= ls[sl] && (cc[sl] & C_ACTIVE)) {
-                float yc[3];
-                rhs_lds<RY, RX>(m, sY, lw[sl], ly[sl], lx[sl], E, f[sl], yc);
-            }
+    for (int ss = 0; ss < D; ss++) {
+        if (ss > 0) {
+            E1 = (float)S.Ef[3 * (step + ss)];
+            E23 = (float)S.Ef[3 * (step + ss) + 1];
+            E4 = (float)S.Ef[3 * (step + ss) + 2];
         }
-        if (k == 1 && kout) {
-            // stage-1 slope f(y_j) of the tile, read back by the adjoint step
 #pragma unroll
-            for (int sl = 0; sl < G::TS; sl++) {
-                int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
-                if (gy >= S.rows) continue;
-                int kk = gy * S.pitch + gx;
-#pragma unroll
-                for (int v = 0; v < 3; v++) gmem(kout)[v * S.n + kk] = f[sl][v];
-                if (kout2)
-#pragma unroll
-                    for (int v = 0; v < 3; v++) gmem(kout2)[v * S.n + kk] = f[sl][v];
-            }
-            if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
-        }
-        if (k < 4) {
-            const float a = (k == 3) ? dt : 0.5f * dt;
-            const float b = (k == 1) ? dt * (1.0f / 6.0f) : dt * (1.0f / 3.0f);
-            __syncthreads();
+        for (int k = 1; k <= 4; k++) {
+            const int gs = 4 * ss + k;   // stage of the launch
+            const float E = (k == 1) ? E1 : (k == 4 ? E4 : E23);
 #pragma unroll
             for (int sl = 0; sl < NS; sl++) {
-                if (!slot_in_stage<G>(sl, k) || k > ls[sl]) continue;
-#pragma unroll
-                for (int v = 0; v < 3; v++) sY[v][ly[sl]][lx[sl]] = y[sl][v] + a * f[sl][v];
-                if (sl < G::TS) {
-#pragma unroll
-                    for (int v = 0; v < 3; v++) acc[sl][v] = (k == 1) ? b * f[sl][v] : acc[sl][v] + b * f[sl][v];
+                if (!slot_in_stage<G>(sl, gs)) continue;
+                f[sl][0] = f[sl][1] = f[sl][2] = 0.0f;
+                if (gs <= ls[sl] && (cc[sl] & C_ACTIVE)) {
+                    float yc[3];
+                    rhs_lds<RY, RX>(m, sY, lw[sl], ly[sl], lx[sl], E, f[sl], yc);
                 }
             }
-            __syncthreads();
+            if (D == 1 && k == 1 && kout) {
+                // stage-1 slope f(y_j) of the tile, read back by the adjoint step
+#pragma unroll
+                for (int sl = 0; sl < G::TS; sl++) {
+                    int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+                    if (gy >= S.rows) continue;
+                    int kk = gy * S.pitch + gx;
+#pragma unroll
+                    for (int v = 0; v < 3; v++) gmem(kout)[v * S.n + kk] = f[sl][v];
+                    if (kout2)
+#pragma unroll
+                        for (int v = 0; v < 3; v++) gmem(kout2)[v * S.n + kk] = f[sl][v];
+                }
+                if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
+            }
+            if (k < 4) {
+                const float a = (k == 3) ? dt : 0.5f * dt;
+                const float b = (k == 1) ? dt * (1.0f / 6.0f) : dt * (1.0f / 3.0f);
+                __syncthreads();
+#pragma unroll
+                for (int sl = 0; sl < NS; sl++) {
+                    if (!slot_in_stage<G>(sl, gs) || gs > ls[sl]) continue;
+#pragma unroll
+                    for (int v = 0; v < 3; v++) sY[v][ly[sl]][lx[sl]] = y[sl][v] + a * f[sl][v];
+                    if (sl < NA && slot_in_stage<G>(sl, 4 * ss + 4)) {
+#pragma unroll
+                        for (int v = 0; v < 3; v++) acc[sl][v] = (k == 1) ? b * f[sl][v] : acc[sl][v] + b * f[sl][v];
+                    }
+                }
+                __syncthreads();
+            } else if (ss + 1 < D) {
+                // end of an inner step: the new state of every cell that took part in all four of
+                // its stages, in registers and in LDS for the next step's stencils
+                const float b4 = dt * (1.0f / 6.0f);
+                __syncthreads();
+#pragma unroll
+                for (int sl = 0; sl < NA; sl++) {
+                    if (!slot_in_stage<G>(sl, gs) || gs > ls[sl]) continue;
+#pragma unroll
+                    for (int v = 0; v < 3; v++) {
+                        y[sl][v] = y[sl][v] + (acc[sl][v] + b4 * f[sl][v]);
+                        sY[v][ly[sl]][lx[sl]] = y[sl][v];
+                    }
+                }
+                __syncthreads();
+            }
         }
     }
     const float b4 = dt * (1.0f / 6.0f);
@@ -607,6 +644,15 @@ void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
     case 7: hipLaunchKernelGGL((k_fused_fwd<8, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     default: hipLaunchKernelGGL((k_fused_fwd<16, 256, 1>), grid, dim3(256), 0, stream, slots_dev, cur, map); break;
     }
+}
+
+// Two forward-sweep steps per launch (MODE_FWD only) on 64x8 tiles of 512 threads with an 8-cell
+// halo: for launches too small to fill the GPU, where a launch costs more than its arithmetic.
+void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream)
+{
+    const dim3 grid(launch_map_tiles(map));
+    if (grid.x == 0) return;
+    hipLaunchKernelGGL((k_fused_fwd<8, 512, 1, 2>), grid, dim3(512), 0, stream, slots_dev, cur, map);
 }
 
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,

@@ -4,8 +4,10 @@ The fused kernels come in tile configurations (64 x TY cells per workgroup of NT
 1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024, 6 = 64x8/512,
 7 = 64x8/256.  The forward tile never
 changes a result; the adjoint tile regroups the per-tile z.gh partial sums (rounding-level changes),
-so it is fixed per grid size.  Time blocking is one time step per launch: the kernels are bound by
-vector-ALU issue, not by HBM, so deeper temporal blocking would only add halo recomputation.
+so it is fixed per grid size.  Time blocking: at these launch sizes one step per launch (the kernels
+are bound by vector-ALU issue, not by HBM, so a deeper halo only adds recomputation); the library
+takes two forward-sweep steps per launch on its own for launches under 1M cells on the built-in
+forward tile, which a tuned forward tile turns off.
 
     python -m fkadj.autotune --grid 512 --batch 16            # sweep one size, print a table
     python -m fkadj.autotune --sizes 256 384 512 768 1024 --write

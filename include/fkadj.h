@@ -106,8 +106,9 @@ typedef struct fkadj_problem {
     int fwd_tile;                     /* FP32 forward tile: 1 = 64x16 cells / 256 threads, 2 = 64x32 / 256,
                                          3 = 64x32 / 512, 4 = 64x16 / 512, 5 = 64x32 / 1024,
                                          6 = 64x8 / 512, 7 = 64x8 / 256;
-                                         0 -> built-in choice by the cells of the launch (the
-                                         forward tile never changes a result) */
+                                         0 -> built-in choice by the cells of the launch, with
+                                         two forward-sweep steps per launch under 1M cells (the
+                                         forward tile and depth never change a result) */
     int adj_tile;                     /* FP32 adjoint tile, same ids; 0 -> built-in table */
     int lazy_workspace;               /* 1: the evaluation buffers of fkadj_grad / fkadj_forward /
                                          batch calls are allocated on first use (a server gives each

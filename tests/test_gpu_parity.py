@@ -397,3 +397,28 @@ def test_random_small_problems_match_oracle(seed):
             g, Lc, _, _ = inst.grad(A)
         assert relerr(g, g_ref) < tol, (prec, relerr(g, g_ref))
         assert abs(Lc - Lc_ref) < tol * abs(Lc_ref), (prec, Lc, Lc_ref)
+
+
+def test_two_step_forward_launches_change_nothing():
+    """Small groups on the built-in forward tile sweep forward two steps per launch (64x8 tiles, an
+    8-cell halo); an explicit 64x8/512 forward tile keeps one step per launch.  Both give the same
+    gradient and J bit for bit, alone and batched."""
+    _print_case("forward_depth2", sizes="(64,64),(96,128)")
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K = 60, 0.02, 6
+    res = {}
+    for ft in (0, 6):
+        insts, Es = [], []
+        for k, (H, W) in enumerate([(64, 64), (96, 128)]):
+            case = disc_case(H, W, 6 + 2 * k, 30 + k)
+            inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                                  alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32,
+                                  ckpt=20, fwd_tile=ft)
+            inst.set_state(case.y0)
+            insts.append(inst)
+            Es.append(np.random.default_rng(k).uniform(0, 10, K))
+        res[ft] = [insts[0].grad(Es[0])] + fkadj.batch_grad(insts, Es)
+        for inst in insts:
+            inst.close()
+    for a, b in zip(res[0], res[6]):
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1]

@@ -24,86 +24,9 @@
 #pragma clang fp contract(on)
 
 #include "fk_internal.hpp"
+#include "fk_tile_map.hpp"
 
 namespace fk {
-
-// A 64 x TY tile advanced by D time steps per launch: 4D RK stages, a halo of 4D cells.
-template <int TY, int NT_, int D = 1>
-struct Tile {
-    static constexpr int TX = FUSED_TX, H = FUSED_HALO * D, NT = NT_;
-    static constexpr int LAST = 4 * D;               // stages of a launch; the tile's cells take part in all
-    static constexpr int NSH = LAST - 1;             // halo shells computed (shell m: stages 1..LAST-m)
-    static constexpr int RX = TX + 2 * H, RY = TY + 2 * H;
-    static constexpr int RXP = RX + 1;               // odd LDS row pitch: column walks hit distinct banks
-    static constexpr int TS = TX * TY / NT;          // tile cells per thread
-    static_assert(TX * TY % NT == 0, "tile must split evenly");
-    // cells at Manhattan distance m from the tile: 2 TX + 2 TY in the bands, m - 1 per corner
-    __host__ __device__ static constexpr int shell_count(int m) { return 2 * (TX + TY) + 4 * (m - 1); }
-    static constexpr int RC = NSH * 2 * (TX + TY) + 2 * NSH * (NSH - 1);   // halo cells computed
-    static constexpr int NRS = (RC + NT - 1) / NT;                         // halo slots per thread
-    static constexpr int NSLOT = TS + NRS;
-    // shell of the r-th halo cell in the shell-1, shell-2, ... sequence
-    __host__ __device__ static constexpr int depth_of(int r)
-    {
-        int m = 1;
-        while (m < NSH && r >= shell_count(m)) r -= shell_count(m++);
-        return m;
-    }
-    // halo slot j holds sequence numbers NT*j .. NT*(j+1)-1: its smallest shell
-    __host__ __device__ static constexpr int min_depth(int j) { return depth_of(NT * j); }
-};
-
-// Cell of slot `sl` of thread `tid` (LDS coordinates) and the last RK stage it takes part in
-// (LAST for tile cells, LAST-m for a shell-m cell); returns false if the thread owns no cell in
-// that slot.
-template <class G>
-__device__ inline bool slot_cell(int sl, int tid, int &ly, int &lx, int &last_stage)
-{
-    constexpr int TX = G::TX, TY = G::RY - 2 * G::H, H = G::H;
-    if (sl < G::TS) {
-        int lin = tid + G::NT * sl;
-        ly = H + lin / TX;
-        lx = H + lin % TX;
-        last_stage = G::LAST;
-        return true;
-    }
-    const int j = sl - G::TS;
-    int e = (j & 1) ? G::NT * (j + 1) - 1 - tid : G::NT * j + tid;
-    last_stage = 0;
-    if (e >= G::RC) return false;
-    int m = 1;
-#pragma unroll
-    for (int mm = 1; mm < G::NSH; mm++)
-        if (m == mm && e >= G::shell_count(mm)) {
-            e -= G::shell_count(mm);
-            m = mm + 1;
-        }
-    last_stage = G::LAST - m;
-    if (e < 2 * TX) {                      // bands above and below the tile
-        const bool below = e >= TX;
-        ly = below ? H + TY - 1 + m : H - m;
-        lx = H + (below ? e - TX : e);
-    } else if (e < 2 * (TX + TY)) {        // bands left and right of the tile
-        const int e2 = e - 2 * TX;
-        const bool right = e2 >= TY;
-        ly = H + (right ? e2 - TY : e2);
-        lx = right ? H + TX - 1 + m : H - m;
-    } else {                               // corner q, offsets a (rows) and m - a (columns)
-        const int c = e - 2 * (TX + TY);
-        const int q = c / (m - 1), a = c - q * (m - 1) + 1;
-        ly = (q & 2) ? H + TY - 1 + a : H - a;
-        lx = (q & 1) ? H + TX - 1 + (m - a) : H - (m - a);
-    }
-    return true;
-}
-
-// Some cell of slot sl may take part in stage k (1..LAST): tile slots always, a halo slot if its
-// smallest shell m has m <= LAST-k (which of its cells do is decided per thread).
-template <class G>
-__device__ constexpr bool slot_in_stage(int sl, int k)
-{
-    return sl < G::TS || G::min_depth(sl - G::TS) <= G::LAST - k;
-}
 
 // Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0 in
 // LDS, so the mirror Laplacian (a missing neighbour takes the opposite one's value) is
@@ -177,9 +100,7 @@ __device__ inline void jtz(const Jac32 &J, const float *z, float *o)
     o[2] = J.a20 * z[0] + J.a22 * z[2];
 }
 
-// Launch tile of this workgroup.  Workgroups are dealt round-robin over the GPU's 8 XCDs (block b
-// runs on XCD b % 8, each XCD with its own L2); XCD x takes a contiguous run of launch tiles, so
-// tiles that share halo lines share an L2.  The map is a bijection for any grid size.
+// Launch tile of this workgroup (XCD-aware order: launch_tile_index).
 struct TileRef {
     int slot, x0, y0, tile;   // instance slot, tile origin (cells), tile index within the instance
 };
@@ -187,39 +108,23 @@ struct TileRef {
 template <int TY>
 __device__ inline TileRef launch_tile(const FusedSlot *__restrict__ slots, const LaunchMap &map)
 {
-    const int n = map.first[map.count];
-    const int b = blockIdx.x, x = b & 7;
-    int lin = b >> 3;
-    for (int q = 0; q < x; q++) lin += (n - q + 7) >> 3;
-    int lo = 0, hi = map.count - 1;   // last i with first[i] <= lin
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (map.first[mid] <= lin) lo = mid;
-        else hi = mid - 1;
-    }
     TileRef t;
-    t.slot = map.slot[lo];
-    t.tile = lin - map.first[lo];
+    launch_tile_index(blockIdx.x, map, t.slot, t.tile);
     const int ntx = slots[t.slot].pitch / FUSED_TX;
     t.x0 = (t.tile % ntx) * FUSED_TX;
     t.y0 = (t.tile / ntx) * TY;
     return t;
 }
 
-// Region (tile + halo of H cells) into LDS.  Each wavefront takes whole rows: lanes 0..63 load the
-// first 64 cells of a row (coalesced), the last 2H cells of 64/(2H) rows per pass.  Cells outside
-// the grid read as 0 with code 0 (inactive).  All loads are issued before the first LDS store
+// Region (tile + halo of H cells) into LDS, by RegionMap: lanes 0..63 load the first 64 cells of a
+// row (coalesced), then the last 2H cells of 64/(2H) rows per pass.  Cells outside the grid read as
+// 0 with code 0 (inactive).  All loads are issued before the first LDS store
 // (their addresses are clamped into the grid, so every load is unconditional and in bounds).
 template <class G>
 __device__ inline void load_region(const FusedSlot &S, const float *__restrict__ src, int x0, int y0, int tid,
                                    float (*dst)[G::RY][G::RXP], uint16_t (*sc)[G::RXP])
 {
-    constexpr int RY = G::RY, NW = G::NT / 64;
-    constexpr int TW = G::RX - 64, RPW = 64 / TW;       // tail width, tail rows per wavefront pass
-    static_assert(TW * RPW == 64, "the tail width must divide a wavefront");
-    constexpr int NR = (RY + NW - 1) / NW;              // rows per wavefront (first 64 cells)
-    constexpr int NTL = (RY + RPW * NW - 1) / (RPW * NW);   // tail passes
-    constexpr int NE = NR + NTL;
+    constexpr int RY = G::RY, NE = RegionMap<G>::NE;
     const int lane = tid & 63, wave = tid >> 6;
     const int gx0 = x0 - G::H, gy0 = y0 - G::H;
     const auto *g = gmem(src);
@@ -230,8 +135,7 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
     uint16_t cv[NE];
 #pragma unroll
     for (int e = 0; e < NE; e++) {
-        if (e < NR) { lyv[e] = wave + e * NW; lxv[e] = lane; }
-        else { lyv[e] = (e - NR) * RPW * NW + wave * RPW + lane / TW; lxv[e] = 64 + lane % TW; }
+        RegionMap<G>::pos(e, lane, wave, lyv[e], lxv[e]);
         const int gy = gy0 + lyv[e], gx = gx0 + lxv[e];
         in[e] = lyv[e] < RY && (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
         const int k = in[e] ? gy * S.pitch + gx : 0;

@@ -73,5 +73,19 @@ def build_c_client() -> str:
     return out
 
 
+def build_map_check() -> str:
+    """tests/tile_map_check.cpp: the fused kernels' index maps (csrc/fk_tile_map.hpp) checked on the host."""
+    root = os.path.dirname(os.path.dirname(PKG))
+    src = os.path.join(root, "tests", "tile_map_check.cpp")
+    out_dir = os.path.join(root, "tests", "_bin")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "tile_map_check")
+    cmd = [HIPCC, "-O1", "-std=c++17", f"--offload-arch={ARCH}", "-I", CSRC, src, "-o", out]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
+    return out
+
+
 if __name__ == "__main__":
     print(build(verbose=True))

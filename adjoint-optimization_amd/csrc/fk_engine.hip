@@ -17,6 +17,7 @@
 #include <deque>
 #include <map>
 #include <mutex>
+#include <new>
 #include <string>
 #include <vector>
 
@@ -107,6 +108,18 @@ bool all_finite(const double *a, size_t n)
 // An instance is the resident problem: geometry codes, objective matrix, model constants and the
 // initial state.  Everything one gradient evaluation writes lives in an EvalCtx, so several
 // evaluations of one instance (a server's requests) can be in flight at once.
+// Entry points that allocate host memory turn a failed allocation (or any other C++ exception)
+// into an error code: no exception crosses the C ABI.
+#define FKADJ_ENTRY try {
+#define FKADJ_EXIT                                \
+    }                                             \
+    catch (const std::bad_alloc &) {              \
+        return FKADJ_ERR_NOMEM;                   \
+    }                                             \
+    catch (...) {                                 \
+        return FKADJ_ERR_DEVICE;                  \
+    }
+
 struct EvalCtx;
 
 struct fkadj_instance {
@@ -855,6 +868,7 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
 
 int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
 {
+    FKADJ_ENTRY
     if (!p || !out || !p->active_index) return FKADJ_ERR_ARG;
     *out = nullptr;
     if (p->nrows < 4 || p->ncols < 4 || p->nt < 1 || !(p->dt > 0) || !std::isfinite(p->dt)) return FKADJ_ERR_ARG;
@@ -981,6 +995,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     if (hipStreamSynchronize(I->stream) != hipSuccess) return fail(FKADJ_ERR_DEVICE);
     *out = I;
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_destroy(fkadj_instance *I)
@@ -1078,16 +1093,20 @@ int fkadj_get_state(fkadj_instance *I, double *y)
 
 int fkadj_grad(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
 {
+    FKADJ_ENTRY
     if (!I || !E) return FKADJ_ERR_ARG;
     return eval_one(I, E, grad, Lc, L0, LE);
+    FKADJ_EXIT
 }
 
 int fkadj_forward(fkadj_instance *I, const double *E, int nsteps, int snap_every, double *snaps, double *y_final,
                   int advance)
 {
+    FKADJ_ENTRY
     if (!I || nsteps < 0 || snap_every < 0) return FKADJ_ERR_ARG;
     const int nsnap = snap_every > 0 ? fkadj_snapshot_count(nsteps, snap_every) : 0;
     return forward_run(I, E, nsteps, snap_every, nsnap, snaps, y_final, advance);
+    FKADJ_EXIT
 }
 
 int fkadj_snapshot_count(int nsteps, int snap_every)
@@ -1102,6 +1121,7 @@ int fkadj_forward_ref(double dt, const double *Et, int nt, const double *y_ini, 
                       int ncols, int n_num, const double *gh_num, int nstep, int ns, double *ys, double *yf,
                       char *param_str, size_t param_str_cap)
 {
+    FKADJ_ENTRY
     if (!Et || !y_ini || !active_index || !gh_num || !yf || nstep < 1 || ns < 0 || nt < 1) return FKADJ_ERR_ARG;
     if (ns > nt / nstep || (ns > 0 && !ys)) return FKADJ_ERR_ARG;
     fkadj_problem p{};
@@ -1137,10 +1157,12 @@ int fkadj_forward_ref(double dt, const double *Et, int nt, const double *y_ini, 
     unpack(fin.data(), yf);
     if (param_str && param_str_cap) fkadj_param_string(&p.model, param_str, param_str_cap);
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *const *E, int nsteps)
 {
+    FKADJ_ENTRY
     if (count < 1 || count > MAX_SLOTS || !insts || !E || nsteps < 0) return FKADJ_ERR_ARG;
     for (int k = 0; k < count; k++) {
         fkadj_instance *I = insts[k];
@@ -1174,10 +1196,12 @@ int fkadj_batch_forward(int count, fkadj_instance *const *insts, const double *c
     if (hipStreamSynchronize(s) != hipSuccess && !rc) rc = FKADJ_ERR_DEVICE;
     group_free(g);
     return rc;
+    FKADJ_EXIT
 }
 
 int fkadj_stimulate(fkadj_instance *I, const uint8_t *cell_mask, double value)
 {
+    FKADJ_ENTRY
     if (!I || !cell_mask || !std::isfinite(value)) return FKADJ_ERR_ARG;
     if (I->busy) return FKADJ_ERR_BUSY;
     if (!I->state_set) return FKADJ_ERR_STATE;
@@ -1195,6 +1219,7 @@ int fkadj_stimulate(fkadj_instance *I, const uint8_t *cell_mask, double value)
     HIPCHK(hipGetLastError());
     HIPCHK(hipStreamSynchronize(I->stream));
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, const int32_t *active_index, int nrows,
@@ -1202,6 +1227,7 @@ int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, con
                    const int64_t *S_rowind, const double *S_val, double alpha, double *L0, double *LE, double *Lcost,
                    double *gradE, char *param_str, size_t param_str_cap)
 {
+    FKADJ_ENTRY
     if (!Et || !y_ini || !active_index || !gh_num || !gamma || !S_colptr || !S_rowind || !S_val || !gradE)
         return FKADJ_ERR_ARG;
     fkadj_problem p{};
@@ -1229,6 +1255,7 @@ int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, con
     fkadj_destroy(I);
     if (!rc && param_str && param_str_cap) fkadj_param_string(&p.model, param_str, param_str_cap);
     return rc;
+    FKADJ_EXIT
 }
 
 int fkadj_set_profiling(int on)
@@ -1248,6 +1275,7 @@ int fkadj_last_timing(fkadj_timing *t)
 int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *const *E, double *const *grad,
                      double *Lcost, double *L0, double *LE)
 {
+    FKADJ_ENTRY
     if (count < 1 || count > MAX_SLOTS || !insts || !E) return FKADJ_ERR_ARG;
     for (int k = 0; k < count; k++) {
         fkadj_instance *I = insts[k];
@@ -1282,6 +1310,7 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
     (void)hipStreamSynchronize(s);
     group_free(g);
     return rc;
+    FKADJ_EXIT
 }
 
 }  // extern "C"
@@ -1333,6 +1362,7 @@ extern "C" {
 
 int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out)
 {
+    FKADJ_ENTRY
     if (!out) return FKADJ_ERR_ARG;
     *out = nullptr;
     CHK(batch_validate(count, insts, nullptr));
@@ -1368,11 +1398,13 @@ int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out)
     }
     *out = P;
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_plan_grad(fkadj_plan *P, const double *const *E, double *const *grad, double *Lcost, double *L0,
                     double *LE)
 {
+    FKADJ_ENTRY
     if (!P || !E) return FKADJ_ERR_ARG;
     const int count = (int)P->insts.size();
     CHK(batch_validate(count, P->insts.data(), E));
@@ -1392,6 +1424,7 @@ int fkadj_plan_grad(fkadj_plan *P, const double *const *E, double *const *grad, 
     }
     record_timing(smp, e0, e1, cells);
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_plan_destroy(fkadj_plan *P)
@@ -1650,6 +1683,7 @@ extern "C" {
 
 int fkadj_server_create(int64_t budget_bytes, int max_slots, fkadj_server **out)
 {
+    FKADJ_ENTRY
     if (!out || budget_bytes <= 0 || max_slots < 1 || max_slots > MAX_SLOTS) return FKADJ_ERR_ARG;
     auto *S = new (std::nothrow) fkadj_server();
     if (!S) return FKADJ_ERR_NOMEM;
@@ -1670,6 +1704,7 @@ int fkadj_server_create(int64_t budget_bytes, int max_slots, fkadj_server **out)
     }
     *out = S;
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_server_destroy(fkadj_server *S)
@@ -1700,6 +1735,7 @@ int fkadj_server_destroy(fkadj_server *S)
 
 int fkadj_server_submit(fkadj_server *S, fkadj_instance *I, const double *E, int64_t *request_id)
 {
+    FKADJ_ENTRY
     if (!S || !I || !E || !request_id) return FKADJ_ERR_ARG;
     if (I->prec != FKADJ_F32) return FKADJ_ERR_ARG;
     if (!I->state_set) return FKADJ_ERR_STATE;
@@ -1714,6 +1750,7 @@ int fkadj_server_submit(fkadj_server *S, fkadj_instance *I, const double *E, int
     I->busy++;   // released when the request finishes, fails or is cancelled
     *request_id = r->id;
     return FKADJ_OK;
+    FKADJ_EXIT
 }
 
 int fkadj_server_cancel(fkadj_server *S, int64_t id)
@@ -1736,6 +1773,7 @@ int fkadj_server_cancel(fkadj_server *S, int64_t id)
 
 int fkadj_server_run(fkadj_server *S, int max_rounds)
 {
+    FKADJ_ENTRY
     if (!S || max_rounds < 0) return FKADJ_ERR_ARG;
     int completed = 0;
     for (int round = 0; max_rounds == 0 || round < max_rounds; round++) {
@@ -1758,6 +1796,7 @@ int fkadj_server_run(fkadj_server *S, int max_rounds)
     }
     completed += server_reap(S, false);
     return completed;
+    FKADJ_EXIT
 }
 
 int fkadj_server_poll(fkadj_server *S, int64_t id, double *grad, double *Lcost, double *L0, double *LE)

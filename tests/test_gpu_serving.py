@@ -136,3 +136,29 @@ def test_queued_request_holds_its_instance():
             srv.run(0)
         srv.poll(rid)
     inst.close()
+
+
+def test_knot_mode_requests_equal_alone():
+    """The reference's own field form (nt+1 knot samples, not-a-knot spline): served and batched
+    requests give the instance's own result bit for bit."""
+    _print_case("server_knots")
+    m = fkadj.reference_model(h=0.025)
+    pairs = []
+    for k, (H, W, nt) in enumerate([(48, 64, 40), (64, 96, 40)]):
+        case = disc_case(H, W, 4 + k, 50 + k)
+        inst = fkadj.Instance(case.ai, H, W, dt=0.02, nt=nt, model=m, pulse_mode=fkadj.PULSE_KNOTS, alpha=2e-3,
+                              gamma=(1, 0.2, 0.2), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=8)
+        inst.set_state(case.y0)
+        Et = 2.0 + np.sin(np.arange(nt + 1) * (0.1 + 0.05 * k))
+        pairs.append((inst, Et))
+    alone = [inst.grad(E) for inst, E in pairs]
+    batched = fkadj.batch_grad([p[0] for p in pairs], [p[1] for p in pairs])
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=4) as srv:
+        ids = [srv.submit(inst, E) for inst, E in pairs]
+        while srv.pending():
+            srv.run(0)
+        served = [srv.poll(i) for i in ids]
+    for a, b, c in zip(alone, batched, served):
+        assert np.array_equal(a[0], b[0]) and np.array_equal(a[0], c[0]) and a[1] == b[1] == c[1]
+    for inst, _ in pairs:
+        inst.close()

@@ -80,7 +80,7 @@ def build_map_check() -> str:
     out_dir = os.path.join(root, "tests", "_bin")
     os.makedirs(out_dir, exist_ok=True)
     out = os.path.join(out_dir, "tile_map_check")
-    cmd = [HIPCC, "-O1", "-std=c++17", f"--offload-arch={ARCH}", "-I", CSRC, src, "-o", out]
+    cmd = [HIPCC, *FLAGS, "-I", CSRC, src, "-o", out]
     r = subprocess.run(cmd, capture_output=True, text=True)
     if r.returncode != 0:
         raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")

@@ -10,6 +10,8 @@ CHECK = os.path.join(ROOT, "tests", "_bin", "tile_map_check")
 
 
 def test_tile_maps():
-    assert os.path.exists(CHECK), "run __graft_entry__.build() first"
+    if not os.path.exists(CHECK):   # a host program: hipcc builds it without a GPU
+        from fkadj import build as fb
+        fb.build_map_check()
     r = subprocess.run([CHECK], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "tile maps ok" in r.stdout, r.stderr

@@ -446,3 +446,27 @@ def test_fp32_c5_grid_short_horizon_error_bound():
     eJ = abs(out[fkadj.F32][1] / out[fkadj.F64][1] - 1)
     print(f"c5 grid T=100 fp32 vs fp64: grad {e:.3e}, J {eJ:.3e}")
     assert e < 1e-4 and eJ < 1e-4
+
+
+def test_batch_forward_equals_forward_alone():
+    """fkadj_batch_forward advances each resident state exactly as the instance's own forward run
+    with advance (shared launches; the forward tile follows the launch size and changes nothing)."""
+    _print_case("batch_forward", sizes="(64,64),(96,128),(40,192)")
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K, nsteps = 60, 0.02, 6, 45
+    a, b, Es = [], [], []
+    for k, (H, W) in enumerate([(64, 64), (96, 128), (40, 192)]):
+        case = disc_case(H, W, 6 + 2 * k, 40 + k)
+        for lst in (a, b):
+            inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                                  alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=20)
+            inst.set_state(case.y0)
+            lst.append(inst)
+        Es.append(np.random.default_rng(k).uniform(0, 10, K))
+    fkadj.batch_forward(a, Es, nsteps)
+    for inst, E in zip(b, Es):
+        inst.forward(E, nsteps=nsteps, advance=True)
+    for x, y in zip(a, b):
+        assert np.array_equal(x.get_state(), y.get_state())
+    for inst in a + b:
+        inst.close()

@@ -470,3 +470,40 @@ def test_batch_forward_equals_forward_alone():
         assert np.array_equal(x.get_state(), y.get_state())
     for inst in a + b:
         inst.close()
+
+
+def test_stimulus_and_launch_accounting():
+    """fkadj_stimulate sets u on the masked active cells only; with profiling on, the last timing
+    reports the launches of the evaluation: nt forward-sweep steps (two per launch for small
+    groups on the built-in forward tile, one per launch with an explicit tile), nt recompute and nt
+    adjoint launches."""
+    _print_case("stimulus_timing", shape=(48, 80))
+    H, W, nt, K = 48, 80, 40, 4
+    case = disc_case(H, W, 4, 61)
+    m = fkadj.reference_model(h=0.025)
+    counts = {}
+    for ft in (0, 6):
+        with fkadj.Instance(case.ai, H, W, dt=0.02, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                            alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=10,
+                            fwd_tile=ft) as inst:
+            # values exact in FP32, so the resident state reads back unchanged
+            y = np.concatenate([np.zeros(case.n_act), np.ones(case.n_act), np.full(case.n_act, 0.75)])
+            inst.set_state(y)
+            mask = np.zeros((H, W), bool)
+            mask[:, : W // 3] = True
+            inst.stimulate(mask, 1.0)
+            got = inst.get_state()
+            stim = mask.ravel()[case.ai > 0]          # packed order: active cells in increasing k
+            assert np.array_equal(got[: case.n_act], np.where(stim, 1.0, 0.0))
+            assert np.array_equal(got[case.n_act:], y[case.n_act:])
+            fkadj.set_profiling(True)
+            try:
+                inst.grad(np.full(K, 2.0))
+                t = fkadj.last_timing()
+            finally:
+                fkadj.set_profiling(False)
+            counts[ft] = (t["fwd_launches"], t["adj_launches"], t["cell_steps"])
+            assert t["eval_ms"] > 0 and t["fwd_kernel_ms"] > 0 and t["adj_kernel_ms"] > 0
+    pitch = (W + 63) // 64 * 64
+    assert counts[0] == (nt // 2 + nt, nt, H * pitch * nt)
+    assert counts[6] == (2 * nt, nt, H * pitch * nt)

@@ -325,8 +325,12 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 {
     using G = Tile<TY, NT>;
     constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
-    __shared__ float sA[3][RY][RX];   // z, then the stage values Z_k
-    __shared__ float sB[3][RY][RX];   // y(t_{i-1} + dt/3)
+    // Each stage writes its values to a buffer that no stage of the same phase reads, so one
+    // barrier per stage suffices: z -> sA; y(t_{i-1} + dt/3) -> sB; Z_1 -> sX; Z_2 -> sA (z is read
+    // only in stage 1); Z_3 -> sX (Z_1 is read only in stage 2).
+    __shared__ float sA[3][RY][RX];
+    __shared__ float sB[3][RY][RX];
+    __shared__ float sX[3][RY][RX];
     __shared__ uint16_t sC[RY][RX];
 
     const TileRef t = launch_tile<TY>(slots, map);
@@ -413,11 +417,10 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     }
 
     // phase 1b: adjoint stage 1 on R1 with J(y_i)
-    float z[NS][3], Zk[NS][3], dz[TS][3];
+    float z[NS][3], dz[TS][3];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        float lt[3], o[3];
-        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
+        float lt[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
         z[sl][0] = z[sl][1] = z[sl][2] = 0.0f;
         if (sl < TS) dz[sl][0] = dz[sl][1] = dz[sl][2] = 0.0f;
         if (!own[sl]) continue;
@@ -429,28 +432,22 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 float fz = lt[v] + o[v];
-                Zk[sl][v] = z[sl][v] + 0.5f * dt * fz;
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
                 if (sl < TS) dz[sl][v] = (1.0f / 6.0f) * dt * fz;
             }
         }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
     }
-    __syncthreads();
-#pragma unroll
-    for (int sl = 0; sl < NS; sl++)
-        if (own[sl])
-#pragma unroll
-            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
 
     // phase 2: midpoint state yh, J(yh), adjoint stage 2 on R2
     Jac32 Jh[NS];
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        if (!slot_in_stage<G>(sl, 2)) continue;
-        float lt[3], zc[3], o[3];
-        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
-        if (ls[sl] < 2) continue;
-        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
+        if (!slot_in_stage<G>(sl, 2) || ls[sl] < 2) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sX, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             float f3[3], yc[3], yh[3];
             rhs_lds<RY, RX>(m, sB, lap_weights(m, cc[sl]), ly[sl], lx[sl], E3, f3, yc);
@@ -462,43 +459,33 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 float fz = lt[v] + o[v];
-                Zk[sl][v] = z[sl][v] + 0.5f * dt * fz;
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
                 if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
             }
         }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[v];
     }
-    __syncthreads();
-#pragma unroll
-    for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<G>(sl, 2) && ls[sl] >= 2)
-#pragma unroll
-            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
 
     // phase 3: adjoint stage 3 on R3 with J(yh)
 #pragma unroll
     for (int sl = 0; sl < NS; sl++) {
-        if (!slot_in_stage<G>(sl, 3)) continue;
-        float lt[3], zc[3], o[3];
-        Zk[sl][0] = Zk[sl][1] = Zk[sl][2] = 0.0f;
-        if (ls[sl] < 3) continue;
+        if (!slot_in_stage<G>(sl, 3) || ls[sl] < 3) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
         lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             jtz(Jh[sl], zc, o);
 #pragma unroll
             for (int v = 0; v < 3; v++) {
                 float fz = lt[v] + o[v];
-                Zk[sl][v] = z[sl][v] + dt * fz;
+                Zk[v] = z[sl][v] + dt * fz;
                 if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
             }
         }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
     }
-    __syncthreads();
-#pragma unroll
-    for (int sl = 0; sl < NS; sl++)
-        if (slot_in_stage<G>(sl, 3) && ls[sl] >= 3)
-#pragma unroll
-            for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[sl][v];
     __syncthreads();
 
     // phase 4: adjoint stage 4 on the tile with J(y_{i-1}); new z and its z.gh partial
@@ -506,7 +493,7 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 #pragma unroll
     for (int sl = 0; sl < TS; sl++) {
         float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
-        lapT_lds<RY, RX>(m, sA, tw[sl], ly[sl], lx[sl], lt, zc);
+        lapT_lds<RY, RX>(m, sX, tw[sl], ly[sl], lx[sl], lt, zc);
         if (act[sl]) {
             Gates32 g = gates32(m, ya[sl][0]);
             Jac32 J = jac32(m, g, ya[sl][0], ya[sl][1], ya[sl][2]);
@@ -533,7 +520,7 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
 // VGPRs, two spilled dwords) runs 8% faster than at the 6 its 67 VGPRs allow; the 64x8/512 adjoint
 // step fits 6 waves without spilling.  Requests that force larger spills (64x16/512 adjoint at 6
 // waves: 190 us instead of 115 us per 16 x 512^2 launch) are not made; at 5 waves it would spill
-// ten dwords and still hold two workgroups per CU (its 45.5 KB of LDS allow three, which need 6).
+// ten dwords and still hold two workgroups per CU (its 66.6 KB of LDS allow two).
 void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream)
 {

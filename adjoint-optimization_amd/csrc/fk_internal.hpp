@@ -44,6 +44,30 @@ __device__ inline const FK_GLOBAL T *gmem(const T *p) { return (const FK_GLOBAL 
 template <typename T>
 __device__ inline FK_GLOBAL T *gmem(T *p) { return (FK_GLOBAL T *)p; }
 
+// p[i] for a wave-uniform base p and an index whose byte offset fits in 32 bits (instance creation
+// keeps all three planes of a state under 2^31 bytes): the address is the scalar base plus a
+// zero-extended 32-bit vector offset, so an access costs no 64-bit address arithmetic per lane.
+// The same wave-uniform pointer, held in scalar registers and opaque to the compiler's address
+// reassociation: plane bases formed with it stay scalar, so ld_at/st_at on them keep the
+// "scalar base + vector offset" form instead of 64-bit per-lane additions.
+template <typename T>
+__device__ inline T *uniform_ptr(T *p)
+{
+    uint64_t a = (uint64_t)p;
+    asm("" : "+s"(a));
+    return (T *)a;
+}
+template <typename T>
+__device__ inline T ld_at(const FK_GLOBAL T *p, uint32_t i)
+{
+    return *(const FK_GLOBAL T *)((const FK_GLOBAL char *)p + i * (uint32_t)sizeof(T));
+}
+template <typename T>
+__device__ inline void st_at(FK_GLOBAL T *p, uint32_t i, T v)
+{
+    *(FK_GLOBAL T *)((FK_GLOBAL char *)p + i * (uint32_t)sizeof(T)) = v;
+}
+
 // ---- plain path context
 template <typename T, typename M>
 struct PlainCtx {

@@ -138,10 +138,10 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
         RegionMap<G>::pos(e, lane, wave, lyv[e], lxv[e]);
         const int gy = gy0 + lyv[e], gx = gx0 + lxv[e];
         in[e] = lyv[e] < RY && (unsigned)gy < (unsigned)S.rows && (unsigned)gx < (unsigned)S.pitch;
-        const int k = in[e] ? gy * S.pitch + gx : 0;
-        cv[e] = gc[k];
+        const uint32_t k = in[e] ? gy * S.pitch + gx : 0;
+        cv[e] = ld_at(gc, k);
 #pragma unroll
-        for (int v = 0; v < 3; v++) val[e][v] = g[v * S.n + k];
+        for (int v = 0; v < 3; v++) val[e][v] = ld_at(uniform_ptr(g + (size_t)v * S.n), k);
     }
 #pragma unroll
     for (int e = 0; e < NE; e++) {
@@ -253,10 +253,10 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
                     if (gy >= S.rows) continue;
                     int kk = gy * S.pitch + gx;
 #pragma unroll
-                    for (int v = 0; v < 3; v++) gmem(kout)[v * S.n + kk] = f[sl][v];
+                    for (int v = 0; v < 3; v++) st_at(uniform_ptr(gmem(kout) + (size_t)v * S.n), kk, f[sl][v]);
                     if (kout2)
 #pragma unroll
-                        for (int v = 0; v < 3; v++) gmem(kout2)[v * S.n + kk] = f[sl][v];
+                        for (int v = 0; v < 3; v++) st_at(uniform_ptr(gmem(kout2) + (size_t)v * S.n), kk, f[sl][v]);
                 }
                 if (mode == MODE_RHS) return;   // uniform over the block: no barrier is left behind
             }
@@ -300,7 +300,9 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
         if (gy >= S.rows) continue;
         int kk = gy * S.pitch + gx;
 #pragma unroll
-        for (int v = 0; v < 3; v++) gmem(yout)[v * S.n + kk] = (cc[sl] & C_ACTIVE) ? y[sl][v] + (acc[sl][v] + b4 * f[sl][v]) : 0.0f;
+        for (int v = 0; v < 3; v++)
+            st_at(uniform_ptr(gmem(yout) + (size_t)v * S.n), kk,
+                  (cc[sl] & C_ACTIVE) ? y[sl][v] + (acc[sl][v] + b4 * f[sl][v]) : 0.0f);
     }
 }
 

@@ -150,6 +150,7 @@ struct fkadj_instance {
     int nb_plain = 0;     // plain blocks (zgh partial row length of the plain path)
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
     int fwd_cfg = 0, adj_cfg = TILE_16_512;   // fused tile configurations (fwd 0: per group)
+    int keep = 0;         // FP32: last segments the default context keeps from the forward sweep
 
     EvalCtx *ctx = nullptr;   // default context of fkadj_grad / fkadj_forward / batches
 
@@ -165,6 +166,7 @@ struct EvalCtx {
     char *d_kseg = nullptr, *d_kT = nullptr;          // FP32: stored slopes of the recompute sweep
     char *d_kfirst = nullptr;                         // FP32: first slope of each segment, mod 3
     int seg_copies = 1;                               // FP32: segment state / slope buffers (1 or 2)
+    int keep = 0;                                     // FP32: kept segments (buffers of their own)
     FusedSlot slot_h{};
     struct FusedGroupDev *group = nullptr;            // this context alone (FP32)
     std::vector<double> Ef, Ea;                       // host tables of the last evaluation
@@ -185,6 +187,9 @@ struct FusedGroupDev {
     // forward sweep two steps per launch (64x8 tiles, all members) instead of one
     bool depth2 = false;
     LaunchMap fwd2;
+    // segments from keep0 on are kept by every member: the forward sweep fills them and the
+    // backward pass does not recompute them
+    int keep0 = 0;
 };
 
 namespace {
@@ -475,19 +480,29 @@ int seg_copies_of(int prec, int64_t cells, bool own_context)
 }
 
 // States held by an evaluation context.  FP64: checkpoints, one segment, ring, adjoint pair, 6 stage
-// planes; FP32: checkpoints, c segments, ring, adjoint pair, c x K stored slopes, f(y_nt), 3 first
-// slopes.
-int64_t ctx_states(int prec, int64_t nck, int64_t K, int c)
+// planes; FP32: checkpoints, c recompute segments and `keep` kept ones, ring, adjoint pair, K stored
+// slopes per segment buffer, f(y_nt), 3 first slopes.
+int64_t ctx_states(int prec, int64_t nck, int64_t K, int c, int64_t keep = 0)
 {
-    return prec == FKADJ_F64 ? nck + (K + 1) + 2 + 2 + 6 : nck + c * (K + 1) + 2 + 2 + c * K + 1 + 3;
+    return prec == FKADJ_F64 ? nck + (K + 1) + 2 + 2 + 6 : nck + (c + keep) * (2 * K + 1) + 2 + 2 + 1 + 3;
+}
+
+// Segments an FP32 instance keeps from the forward sweep within keep_bytes (each: K+1 states and
+// K slopes), at most all of them.
+int kept_segments(int prec, int64_t keep_bytes, int64_t st_bytes, int K, int nt)
+{
+    if (prec != FKADJ_F32 || keep_bytes <= 0) return 0;
+    const int64_t nseg = (nt + K - 1) / K;
+    return (int)std::min<int64_t>(nseg, keep_bytes / ((2 * (int64_t)K + 1) * st_bytes));
 }
 
 // Device bytes of one evaluation context (tables, partials, checkpoints, segment states, ring,
-// adjoint ping-pong, FP64 stage scratch or FP32 stored slopes) with c segment buffers.
-int64_t ctx_bytes(const fkadj_instance *I, int c)
+// adjoint ping-pong, FP64 stage scratch or FP32 stored slopes) with c recompute segment buffers and
+// `keep` kept segments.
+int64_t ctx_bytes(const fkadj_instance *I, int c, int keep = 0)
 {
     const int64_t st = (int64_t)I->st_bytes;
-    const int64_t states = ctx_states(I->prec, I->nck, I->K, c);
+    const int64_t states = ctx_states(I->prec, I->nck, I->K, c, keep);
     int64_t tables = 8 * (int64_t)(6 * I->nt + (int64_t)I->nt * I->nb_part + I->nb_plain + (I->nt + 1) +
                                    (I->n_num + 255) / 256 + 1);
     return states * st + tables;
@@ -513,6 +528,8 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
     if (!C) return FKADJ_ERR_NOMEM;
     C->I = I;
     C->seg_copies = seg_copies_of(I->prec, (int64_t)I->rows * I->pitch, self_group);
+    C->keep = self_group ? I->keep : 0;   // a server request recomputes every segment
+    const int nbuf = C->seg_copies + C->keep;
     const size_t st = I->st_bytes;
     auto alc = [&](char *&q, size_t bytes) -> bool {
         if (hipMalloc((void **)&q, bytes) != hipSuccess) { q = nullptr; return false; }
@@ -525,17 +542,18 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
     bool ok = ald(C->d_Ef, 3 * (size_t)nt) && ald(C->d_Ea, 3 * (size_t)nt) && ald(C->d_part, (size_t)nt * I->nb_part) &&
               ald(C->d_partT, I->nb_plain) && ald(C->d_zgh, nt + 1) && ald(C->d_L0part, objective_blocks(I->n_num)) &&
               ald(C->d_L0, 1) && alc(C->d_ck, st * I->nck) &&
-              alc(C->d_seg, st * (I->K + 1) * C->seg_copies) &&
+              alc(C->d_seg, st * (I->K + 1) * nbuf) &&
               alc(C->d_ring, st * 2) && alc(C->d_z, st * 2);
     if (ok && I->prec == FKADJ_F64) ok = alc(C->d_work, st * 6);
     if (ok && I->prec == FKADJ_F32)
-        ok = alc(C->d_kseg, st * C->seg_copies * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
+        ok = alc(C->d_kseg, st * nbuf * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
     if (ok && I->prec == FKADJ_F32) {
         FusedSlot &S = C->slot_h;
         S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
         S.nt = I->nt; S.K = I->K; S.nck = I->nck;
         S.ntiles = I->nb_part;
         S.seg_par = C->seg_copies - 1;
+        S.keep0 = (I->nt + I->K - 1) / I->K - C->keep;
         S.code = I->d_code;
         S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
         S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT); S.kfirst = as<float>(C->d_kfirst);
@@ -551,7 +569,7 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
         ctx_free(C);
         return FKADJ_ERR_NOMEM;
     }
-    C->bytes = ctx_bytes(I, C->seg_copies);
+    C->bytes = ctx_bytes(I, C->seg_copies, C->keep);
     *out = C;
     return FKADJ_OK;
 }
@@ -655,9 +673,11 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
     // forward tile, and a step count and checkpoint interval that pair up.
     const fkadj_instance *I0 = members[0]->I;
     g->depth2 = cells < kDepth2Cells && I0->nt % 2 == 0 && I0->K % 2 == 0;
+    g->keep0 = 0;
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k]->I;
         slots.push_back(members[k]->slot_h);
+        g->keep0 = std::max(g->keep0, members[k]->slot_h.keep0);
         const int fc = fwd_cfg_of(I, cells);
         launch_map_add(g->fwd[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
         launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
@@ -707,10 +727,14 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     const int nt = I0->nt, K = I0->K;
     for (auto *C : g->members)
         HIPCHK(hipMemcpyAsync(C->I->state(C->d_ck, 0), C->I->d_y0, C->I->st_bytes, hipMemcpyDeviceToDevice, s));
+    // steps of the kept segments (from keep0 on) store their states and slopes as they go; keep0*K
+    // is a checkpoint, and even where two steps run per launch (K even)
+    const int kept_from = std::min(g->keep0 * K, nt);
     if (g->depth2)
-        for (int i = 0; i < nt; i += 2) smp.fwd_launch(s, [&] { group_forward2(g, i, s); });
+        for (int i = 0; i < kept_from; i += 2) smp.fwd_launch(s, [&] { group_forward2(g, i, s); });
     else
-        for (int i = 0; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
+        for (int i = 0; i < kept_from; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
+    for (int i = kept_from; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_KEEP, i, i / K * K, s); });
     HIPCHK(hipGetLastError());
     for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step
@@ -732,8 +756,11 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt), p = sg & 1;
         if (sg + 2 <= nseg - 1) HIPCHK(hipStreamWaitEvent(s, g->ev_adj[p], 0));
-        // all b-a steps are replayed: the last one only to store the slope f(y_{b-1})
-        for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
+        // all b-a steps are replayed: the last one only to store the slope f(y_{b-1}); kept
+        // segments are not (a member that keeps more segments than the group recomputes some of
+        // its kept ones, into their own buffers, with the same values)
+        if (sg < g->keep0)
+            for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
         HIPCHK(hipEventRecord(g->ev_rec[p], s));
         HIPCHK(hipStreamWaitEvent(side, g->ev_rec[p], 0));
         for (int i = b; i > a; i--) smp.adj_launch(side, [&] { group_adjoint(g, i, a, side); });
@@ -858,12 +885,26 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t esz = p->precision == FKADJ_F32 ? 4 : 8;
     const int64_t st = 3 * n * esz;
     const int64_t nck = (p->nt + K - 1) / K + 1;
-    const int64_t states = 1 + ctx_states(p->precision, nck, K, seg_copies_of(p->precision, n, true));
+    const int64_t states = 1 + ctx_states(p->precision, nck, K, seg_copies_of(p->precision, n, true),
+                                          kept_segments(p->precision, p->keep_bytes, st, K, p->nt));
     const int acfg = p->adj_tile > 0 && p->adj_tile < NUM_TILE_CFG ? p->adj_tile : default_adj_cfg(p->nrows, pitch);
     int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : tiles_of(p->nrows, pitch, acfg);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
                     8 * 3 * n;
     return states * st + extra;
+}
+
+int fkadj_mem_info(int64_t *free_bytes, int64_t *total_bytes)
+{
+    if (!free_bytes || !total_bytes) return FKADJ_ERR_ARG;
+    size_t fr = 0, tot = 0;
+    if (hipMemGetInfo(&fr, &tot) != hipSuccess) {
+        snprintf(g_errmsg, sizeof g_errmsg, "hipMemGetInfo failed (no GPU?)");
+        return FKADJ_ERR_DEVICE;
+    }
+    *free_bytes = (int64_t)fr;
+    *total_bytes = (int64_t)tot;
+    return FKADJ_OK;
 }
 
 int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
@@ -876,7 +917,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     // kernels index a state with 32-bit offsets (bytes for the FP32 buffer loads)
     if ((int64_t)p->nrows * ((p->ncols + 63) / 64 * 64) * 3 * (p->precision == FKADJ_F32 ? 4 : 8) >= (int64_t)1 << 31)
         return FKADJ_ERR_ARG;
-    if (p->fwd_tile < 0 || p->fwd_tile >= NUM_TILE_CFG || p->adj_tile < 0 || p->adj_tile >= NUM_TILE_CFG)
+    if (p->fwd_tile < 0 || p->fwd_tile >= NUM_TILE_CFG || p->adj_tile < 0 || p->adj_tile >= NUM_TILE_CFG ||
+        p->keep_bytes < 0)
         return FKADJ_ERR_ARG;
     if (p->pulse_mode == FKADJ_PULSE_KNOTS) {
         if (p->nt + 1 < 4) return FKADJ_ERR_ARG;
@@ -899,6 +941,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     I->esz = I->prec == FKADJ_F32 ? 4 : 8;
     I->st_bytes = (size_t)3 * I->n * I->esz;
     I->ncontrols = p->pulse_mode == FKADJ_PULSE_KNOTS ? p->nt + 1 : p->n_amp;
+    I->keep = kept_segments(I->prec, p->keep_bytes, (int64_t)I->st_bytes, I->K, I->nt);
     fill_models(I);
 
     std::vector<uint16_t> code;

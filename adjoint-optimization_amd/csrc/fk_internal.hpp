@@ -105,12 +105,15 @@ struct FusedSlot {
     int nt, K, nck;             // steps, checkpoint interval, checkpoints
     int ntiles;                 // tiles of this instance (zgh partial row length)
     int seg_par;                // 1: seg and kseg hold two buffers, chosen by segment parity; 0: one
+    int keep0;                  // first kept segment (segments keep0.. have buffers of their own,
+                                // filled by the forward sweep); the segment count when none is kept
     const uint16_t *code;
     float *ck;                  // nck states: ck[c] = state at step min(c*K, nt)
-    float *seg;                 // (seg_par+1) x (K+1) states of recomputed segments (state 0 unused)
+    float *seg;                 // (seg_par+1+kept) x (K+1) states of recomputed and kept segments
+                                // (state 0 unused)
     float *ring;                // 2 states for the forward sweep between checkpoints
     float *zb;                  // 2 adjoint states (ping-pong)
-    float *kseg;                // (seg_par+1) x K slopes f(y_j) of recomputed steps
+    float *kseg;                // (seg_par+1+kept) x K slopes f(y_j) of recomputed and kept steps
     float *kfirst;              // 3 slopes: f(y_a) of the segment starting at a, by segment index mod 3
     float *kT;                  // slope f(y_nt) at the final time with the adjoint's field value
     const double *Ef, *Ea;      // stage field tables, 3 per step
@@ -120,10 +123,18 @@ struct FusedSlot {
 };
 
 
+// Segment buffer of segment segi: a kept segment's own, else a recompute buffer by parity.
+__host__ __device__ inline int seg_buf(const FusedSlot &S, int segi)
+{
+    return segi >= S.keep0 ? S.seg_par + 1 + (segi - S.keep0) : (segi & S.seg_par);
+}
+
 // MODE_FWD: forward sweep (checkpoints + ring); MODE_REC: segment recompute, also stores the
-// stage-1 slopes for the adjoint's Hermite reconstruction; MODE_RHS: only f(y_nt) -> kT;
-// MODE_ADJ: adjoint step; MODE_IDLE: the slot's tiles do nothing in this launch.
-enum FusedMode { MODE_IDLE = -1, MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2, MODE_ADJ = 3 };
+// stage-1 slopes for the adjoint's Hermite reconstruction; MODE_KEEP: forward-sweep step of a kept
+// segment (states and slopes into the segment's own buffers as MODE_REC, the segment's end state to
+// its checkpoint); MODE_RHS: only f(y_nt) -> kT; MODE_ADJ: adjoint step; MODE_IDLE: the slot's tiles
+// do nothing in this launch.
+enum FusedMode { MODE_IDLE = -1, MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2, MODE_ADJ = 3, MODE_KEEP = 4 };
 
 // Per-slot work of one launch, passed by value (1 KiB of kernel arguments): slots of one launch
 // may be at different steps and phases (continuous batching).  step is the forward step index

@@ -167,7 +167,8 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
 
     const TileRef t = launch_tile<TY>(slots, map);
     const SlotCursor sc = cur.c[t.slot];
-    if (sc.mode != MODE_FWD && (D > 1 || (sc.mode != MODE_REC && sc.mode != MODE_RHS))) return;   // uniform per block
+    if (sc.mode != MODE_FWD && (D > 1 || (sc.mode != MODE_REC && sc.mode != MODE_RHS && sc.mode != MODE_KEEP)))
+        return;   // uniform per block
     const int mode = sc.mode, step = sc.step, seg_a = sc.seg_a;
     const FusedSlot &S = slots[t.slot];
     const int tid = threadIdx.x;
@@ -183,17 +184,19 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
         int nx = step + D;
         yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
                                              : S.ring + (size_t)((nx / D) & 1) * st;
-    } else if (mode == MODE_REC) {
-        // with two buffers (seg_par 1) segments alternate between them, so the recompute of
-        // segment s-1 may run while the adjoint steps of segment s read segment s's buffers; the
-        // first slope of each segment is also kept by segment index mod 3 for the adjoint step
-        // that ends at it (the last one of the previous segment)
-        const int j = step - seg_a, segi = seg_a / S.K;
-        float *segb = S.seg + (size_t)(segi & S.seg_par) * (S.K + 1) * st;
+    } else if (mode == MODE_REC || mode == MODE_KEEP) {
+        // with two buffers (seg_par 1) recomputed segments alternate between them, so the
+        // recompute of segment s-1 may run while the adjoint steps of segment s read segment s's
+        // buffers; the first slope of each recomputed segment is also kept by segment index mod 3
+        // for the adjoint step that ends at it (the last one of the previous segment), which reads
+        // a kept segment's first slope from that segment's own buffer
+        const int j = step - seg_a, segi = seg_a / S.K, buf = seg_buf(S, segi), nx = step + 1;
+        float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
         yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
-        yout = segb + (size_t)(j + 1) * st;
-        kout = S.kseg + ((size_t)(segi & S.seg_par) * S.K + j) * st;
-        if (j == 0) kout2 = S.kfirst + (size_t)(segi % 3) * st;
+        yout = (mode == MODE_KEEP && (nx % S.K == 0 || nx == S.nt)) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
+                                                                   : segb + (size_t)(j + 1) * st;
+        kout = S.kseg + ((size_t)buf * S.K + j) * st;
+        if (mode == MODE_REC && j == 0 && segi < S.keep0) kout2 = S.kfirst + (size_t)(segi % 3) * st;
     } else {
         yin = S.ck + (size_t)(S.nck - 1) * st;
         yout = nullptr;
@@ -344,14 +347,18 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     const size_t st = (size_t)3 * S.n;
     const int segi = seg_a / S.K;
     const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
-    const float *kbuf = S.kseg + (size_t)(segi & S.seg_par) * S.K * st;
-    const float *segb = S.seg + (size_t)(segi & S.seg_par) * (S.K + 1) * st;
+    const int buf = seg_buf(S, segi);
+    const float *kbuf = S.kseg + (size_t)buf * S.K * st;
+    const float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
     const auto *yA = gmem((i - 1 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 1 - seg_a) * st);
     const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st);
     const auto *kA = gmem(kbuf + (size_t)(i - 1 - seg_a) * st);
-    // f(y_i): slope of the next recomputed step, the first slope of the later segment, or f(y_nt)
-    const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st
-                                      : (b_end < S.nt ? S.kfirst + (size_t)((segi + 1) % 3) * st : S.kT));
+    // f(y_i): slope of the next recomputed step, the first slope of the later segment (from its own
+    // buffer when it is kept), or f(y_nt)
+    const float *knext = (b_end == S.nt) ? S.kT
+                         : (segi + 1 >= S.keep0 ? S.kseg + (size_t)seg_buf(S, segi + 1) * S.K * st
+                                                : S.kfirst + (size_t)((segi + 1) % 3) * st);
+    const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext);
     const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
     auto *zout = gmem(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
     const Model32 m = S.m;

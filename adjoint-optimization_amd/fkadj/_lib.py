@@ -40,6 +40,7 @@ class Problem(C.Structure):
         ("s_diag", C.c_double),
         ("precision", C.c_int), ("ckpt", C.c_int), ("fwd_tile", C.c_int), ("adj_tile", C.c_int),
         ("lazy_workspace", C.c_int),
+        ("keep_bytes", C.c_int64),
     ]
 
 
@@ -80,6 +81,7 @@ def lib():
     L.fkadj_footprint_bytes.restype = C.c_int64
     L.fkadj_request_bytes.argtypes = [vp]
     L.fkadj_request_bytes.restype = C.c_int64
+    L.fkadj_mem_info.argtypes = [C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.fkadj_set_state.argtypes = [vp, dp]
     L.fkadj_get_state.argtypes = [vp, dp]
     L.fkadj_n_active.argtypes = [vp]

@@ -86,11 +86,14 @@ def fk_forward_Et_mex(Et, dt, nt, y0, active_index, nrows, ncols, ghost_lr, nste
 
 
 class Instance:
-    """A device-resident gradient problem (one grid, one objective, one resident state)."""
+    """A device-resident gradient problem (one grid, one objective, one resident state).
+
+    keep_bytes (FP32): device bytes for keeping the last checkpoint segments of the forward
+    trajectory, which the backward pass then does not recompute (same results, fewer launches)."""
 
     def __init__(self, active_index, nrows, ncols, *, dt, nt, model=None, ghost_lr=None, pulse_mode=L.PULSE_KNOTS,
                  n_amp=0, alpha=0.5, gamma=(1.0, 0.2, 0.2), S=None, s_diag=0.0, precision=L.F32, ckpt=50,
-                 fwd_tile=0, adj_tile=0, lazy_workspace=False):
+                 fwd_tile=0, adj_tile=0, lazy_workspace=False, keep_bytes=0):
         self._keep = []
         ai = np.ascontiguousarray(np.ravel(active_index), dtype=np.int32)
         p = L.Problem()
@@ -116,6 +119,7 @@ class Instance:
         p.s_diag = float(s_diag)
         p.precision, p.ckpt, p.fwd_tile, p.adj_tile = int(precision), int(ckpt), int(fwd_tile), int(adj_tile)
         p.lazy_workspace = 1 if lazy_workspace else 0
+        p.keep_bytes = int(keep_bytes)
         self.problem = p
         self.nrows, self.ncols, self.nt, self.dt = p.nrows, p.ncols, p.nt, p.dt
         self.precision = p.precision
@@ -307,3 +311,10 @@ class Server:
 
 def footprint_bytes(problem: L.Problem) -> int:
     return int(L.lib().fkadj_footprint_bytes(C.byref(problem)))
+
+
+def mem_info() -> tuple[int, int]:
+    """(free, total) device bytes of the current GPU."""
+    fr, tot = C.c_int64(), C.c_int64()
+    check(L.lib().fkadj_mem_info(C.byref(fr), C.byref(tot)), "fkadj_mem_info")
+    return int(fr.value), int(tot.value)

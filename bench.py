@@ -135,7 +135,9 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
     else:
         dom, dbytes, dms, dkey = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms, "fwd"
     achieved = dbytes / (dms * 1e-3) / 1e9
-    eval_bytes = algorithmic_bytes_per_cell_step(ckpt) * cells * nt
+    # compulsory words of the launches of one evaluation (kept segments are not recomputed, so the
+    # forward launches count nt sweep steps plus the recomputed ones)
+    eval_bytes = 4 * cells * (7 * nf + 13 * na)
 
     def valu(key, ms):
         rate = VALU_LANE_INSTR_PER_CELL[key] / 64 * cells / (ms * 1e-3) / 1e9
@@ -463,6 +465,8 @@ def main():
     ap.add_argument("--skip-plain", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-evals", type=int, default=4)
+    ap.add_argument("--no-keep", action="store_true",
+                    help="C2: recompute every segment (no trajectory segments kept in spare HBM)")
     ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
                     help="c2 (default): 16 x C2 batched; c1: 128^2 FP64 with the CPU oracle; c3: 64 uneven "
                          "grids; c4: served Poisson stream; c5: 2048^2, 25 000 steps")
@@ -501,9 +505,24 @@ def main():
     from fkadj import autotune
     _, adj_tile = autotune.lookup(N)   # the forward tile follows each launch's size (built-in rule)
     fwd_tile = 0
+    # spare HBM keeps the last trajectory segments of every instance (no recompute for them): what
+    # is left after the batch's own buffers, a single-instance FP32 and an FP64 instance, and 16 GB
+    pr = fkadj.Problem()
+    pr.nrows = pr.ncols = N
+    pr.nt, pr.ckpt, pr.precision, pr.adj_tile = nt, ckpt, fkadj.F32, adj_tile
+    fp32 = fkadj.footprint_bytes(pr)
+    pr.precision = fkadj.F64
+    fp64 = fkadj.footprint_bytes(pr)
+    free0, total0 = fkadj.mem_info()
+    keep = 0 if args.no_keep else max(0, (free0 - (B + 1) * fp32 - fp64 - (16 << 30)) // B)
+    seg_bytes = (2 * ckpt + 1) * 3 * 4 * N * ((N + 63) // 64 * 64)
+    kept = min(keep // seg_bytes, (nt + ckpt - 1) // ckpt)
+    log(f"[bench] {free0 / 1e9:.0f} of {total0 / 1e9:.0f} GB free; keeping {kept} of {(nt + ckpt - 1) // ckpt} "
+        f"segments per instance ({kept * seg_bytes / 1e9:.1f} GB each)")
     insts = []
     for m, y in zip(masks, states):
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile,
+                               keep_bytes=keep)
         inst.set_state(y)
         insts.append(inst)
 
@@ -524,14 +543,19 @@ def main():
     fkadj.set_profiling(False)
     value = B * args.steps / total
 
-    # single-instance FP32 latency
+    # instance 0 alone (its kept segments too), then single-instance FP32 latency on an instance
+    # that keeps no segments (a lone 512^2 instance overlaps its recompute with the adjoint on a
+    # second stream, so keeping gains it nothing): all three results are the same bit for bit
+    g_alone = insts[0].grad(A[0])
+    ish = W.make_instance(masks[0], T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)
+    ish.set_state(states[0])
     lat = []
     for k in range(args.latency_evals):
         ts = time.perf_counter()
-        g32_single = insts[0].grad(A[0])
+        g32_single = ish.grad(A[0])
         lat.append(1e3 * (time.perf_counter() - ts))
-    g_alone = g32_single[0]
-    bit_identical = bool(np.array_equal(g_alone, res[0][0]) and g32_single[1] == res[0][1])
+    ish.close()
+    bit_identical = bool(all(np.array_equal(g[0], res[0][0]) and g[1] == res[0][1] for g in (g_alone, g32_single)))
     log(f"[bench] single FP32 eval {np.median(lat):.1f} ms; alone == batched: {bit_identical}")
 
     # plain FP64 single instance: the baseline path and the FP64 reference of instance 0
@@ -567,7 +591,8 @@ def main():
         "vs_baseline": None, "dtype": "f32", "data": "synthetic",
         "config": {"workload": f"C2 x{B} batched: {N}^2 FP32, {holes} holes, T={T:g} ms, dt={W.DT_MS} ms, "
                                f"{K_amp} amplitudes, checkpoint every {ckpt} steps, S1-S2 spirals",
-                   "grid": N, "batch": B, "nt": nt, "n_amp": K_amp, "ckpt": ckpt},
+                   "grid": N, "batch": B, "nt": nt, "n_amp": K_amp, "ckpt": ckpt,
+                   "kept_segments": int(kept), "keep_bytes_per_instance": int(keep)},
         "plain_f64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
         "rel_err_vs_f64": rel,

@@ -114,6 +114,13 @@ typedef struct fkadj_problem {
                                          batch calls are allocated on first use (a server gives each
                                          request its own, so an instance used only through a server
                                          holds just its resident data) */
+    int64_t keep_bytes;               /* FP32: device bytes the instance may spend on keeping the last
+                                         checkpoint segments of the forward trajectory (K+1 states and
+                                         K stage-1 slopes each); the forward sweep of fkadj_grad and the
+                                         batch calls fills them and the backward pass recomputes only
+                                         the segments before them.  Results are unchanged.  0: none
+                                         (every segment is recomputed); FP64 and server requests keep
+                                         none */
 } fkadj_problem;
 
 typedef struct fkadj_instance fkadj_instance;
@@ -129,6 +136,9 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p);
  * buffers), the figure the server checks against its budget; negative on a bad argument.  No
  * reference counterpart. */
 int64_t fkadj_request_bytes(const fkadj_instance *inst);
+/* Free and total device memory of the current GPU (bytes), for sizing keep_bytes and server
+ * budgets.  No reference counterpart. */
+int fkadj_mem_info(int64_t *free_bytes, int64_t *total_bytes);
 
 /* y_packed: 3*n_num values [u; v; w].  Stays resident until replaced. */
 int fkadj_set_state(fkadj_instance *inst, const double *y_packed);

@@ -74,6 +74,7 @@ def test_validation_errors_before_device_work():
     _expect(L.ERR_ARG, ai=ai, shape=(16, 16), ghost_lr=np.full(256, 0.5))
     _expect(L.ERR_ARG, ai=ai, shape=(3, 16), dt=0.1)
     _expect(L.ERR_ARG, ai=ai, shape=(16, 16), dt=-1.0)
+    _expect(L.ERR_ARG, ai=ai, shape=(16, 16), keep_bytes=-1)
 
 
 def test_footprint_grows_with_size():
@@ -103,6 +104,23 @@ def test_footprint_one_segment_buffer_from_2M_cells():
     two = fkadj.footprint_bytes(p)
     state = 3 * 4 * 1023 * 2048
     assert 100 * state < two - one < 102 * state
+
+
+def test_footprint_counts_kept_segments():
+    """Each kept segment holds K+1 states and K slopes of its own; a budget buys whole segments, at
+    most every segment of the run (FP64 keeps none)."""
+    p = L.Problem()
+    p.nrows, p.ncols = 256, 200            # pitch 256
+    p.nt, p.ckpt, p.precision = 1030, 50, L.F32   # 21 segments, the last of 30 steps
+    base = fkadj.footprint_bytes(p)
+    seg = (2 * 50 + 1) * 3 * 4 * 256 * 256
+    for budget, kept in ((seg - 1, 0), (seg, 1), (5 * seg + seg // 2, 5), (21 * seg, 21), (1 << 50, 21)):
+        p.keep_bytes = budget
+        assert fkadj.footprint_bytes(p) - base == kept * seg, (budget, kept)
+    p.precision = L.F64
+    with_budget = fkadj.footprint_bytes(p)
+    p.keep_bytes = 0
+    assert fkadj.footprint_bytes(p) == with_budget
 
 
 def test_disc_holes_have_no_isthmus_and_closed_form_ghost():

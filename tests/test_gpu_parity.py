@@ -424,6 +424,51 @@ def test_two_step_forward_launches_change_nothing():
         assert np.array_equal(a[0], b[0]) and a[1] == b[1]
 
 
+def test_kept_segments_change_nothing():
+    """Kept segments (keep_bytes) are filled by the forward sweep and not recomputed: every budget,
+    from none to all segments, gives the same gradient and J bit for bit, alone and batched with
+    members keeping different numbers of segments; the evaluation launches nt forward steps plus the
+    recompute of the segments not kept.  Knot-sampled field; nt = 62 with K = 20 (a short last
+    segment, two forward steps per launch before the kept segments) and nt = 61 (one step per
+    launch)."""
+    m = fkadj.reference_model(h=0.025)
+    dt, K = 0.02, 20
+    shapes = [(64, 64), (96, 128)]
+    for nt in (62, 61):
+        _print_case("kept_segments", sizes="(64,64),(96,128)", nt=nt, K=K)
+        nseg = (nt + K - 1) // K
+        base, insts = {}, {}
+        for k, (H, W) in enumerate(shapes):
+            case = disc_case(H, W, 6 + 2 * k, 40 + k)
+            seg = (2 * K + 1) * 3 * 4 * H * ((W + 63) // 64 * 64)
+            for kept in (0, 1, 2, nseg):
+                inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, alpha=2e-3, gamma=(1, 0, 0),
+                                      s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=K,
+                                      keep_bytes=kept * seg + (seg // 2 if kept < nseg else 1 << 40))
+                inst.set_state(case.y0)
+                insts[k, kept] = inst
+            E = _Et(nt, k)
+            base[k] = insts[k, 0].grad(E)
+            for kept in (1, 2, nseg):
+                fkadj.set_profiling(True)
+                try:
+                    g = insts[k, kept].grad(E)
+                    t = fkadj.last_timing()
+                finally:
+                    fkadj.set_profiling(False)
+                assert np.array_equal(g[0], base[k][0]) and g[1] == base[k][1], (nt, k, kept)
+                recomputed = max(0, min((nseg - kept) * K, nt))
+                sweep = recomputed // 2 + (nt - recomputed) if nt % 2 == 0 else nt
+                assert t["fwd_launches"] == sweep + recomputed and t["adj_launches"] == nt, (nt, k, kept, t)
+        # batched: members keeping 1 and all segments, and none and 2
+        for pair in (((0, 1), (1, nseg)), ((0, 0), (1, 2))):
+            res = fkadj.batch_grad([insts[p] for p in pair], [_Et(nt, p[0]) for p in pair])
+            for (k, _), r in zip(pair, res):
+                assert np.array_equal(r[0], base[k][0]) and r[1] == base[k][1], (nt, pair)
+        for inst in insts.values():
+            inst.close()
+
+
 @pytest.mark.slow
 def test_fp32_c5_grid_short_horizon_error_bound():
     """The C5 grid at its full size (2048^2, 400 disc holes, four interacting spirals) over the first

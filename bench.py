@@ -122,6 +122,24 @@ def oracle_piecewise(mask, y0, A, nt):
     return g, 0.5 * L0 + 0.5 * alpha * LE, sec
 
 
+def footprint(nrows, ncols, nt, ckpt, precision, adj_tile=0):
+    """Device bytes of an instance of this shape that keeps no segments (fkadj_footprint_bytes)."""
+    import fkadj
+    pr = fkadj.Problem()
+    pr.nrows, pr.ncols = nrows, ncols
+    pr.nt, pr.ckpt, pr.precision, pr.adj_tile = nt, ckpt, precision, adj_tile
+    return fkadj.footprint_bytes(pr)
+
+
+def kept_budgets(shapes, nt, ckpt, spare):
+    """(k, keep_bytes per instance): every instance keeps its last k trajectory segments (a group
+    skips the recompute of the segments all of its members keep), k as large as `spare` bytes
+    allow in all."""
+    seg = [(2 * ckpt + 1) * 3 * 4 * r * ((c + 63) // 64 * 64) for r, c in shapes]
+    k = int(min((nt + ckpt - 1) // ckpt, max(0, spare) // sum(seg)))
+    return k, [k * b for b in seg]
+
+
 def roofline_of(timings, cells, ckpt, nt, evals, seconds):
     """Dominant fused kernel's algorithmic-bytes rate from the in-run HIP-event samples."""
     fwd_ms = float(np.mean([t["fwd_kernel_ms"] for t in timings]))
@@ -229,7 +247,13 @@ def run_c5(args):
     from fkadj import autotune
     _, at = autotune.lookup(N)
     ft = 0
-    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=ft, adj_tile=at)
+    # spare HBM (all but the instance's own buffers and 16 GB; the FP64 instance comes after this
+    # one is closed) keeps the last trajectory segments
+    free0, _ = fkadj.mem_info()
+    spare = 0 if args.no_keep else free0 - footprint(N, N, nt, ckpt, fkadj.F32, at) - (16 << 30)
+    kept, (keep,) = kept_budgets([(N, N)], nt, ckpt, spare)
+    log(f"[c5] keeping {kept} of {(nt + ckpt - 1) // ckpt} segments ({keep / 1e9:.0f} GB)")
+    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=ft, adj_tile=at, keep_bytes=keep)
     inst.set_state(y0)
     for _ in range(args.warmup):
         inst.grad(A)
@@ -279,7 +303,7 @@ def run_c5(args):
            "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"C5: {N}^2, {cfg['holes']} holes, four S1-S2 spirals, T={T:g} ms ({nt} steps), "
                                   f"{K_amp} amplitudes, uniform checkpoints every {ckpt} steps",
-                      "phase_singularities": ps, "footprint_GB_f32": fp32_bytes / 1e9},
+                      "phase_singularities": ps, "footprint_GB_f32": fp32_bytes / 1e9, "kept_segments": kept},
            "plain_f64": plain, "rel_err_vs_f64": rel, "grad_rel_err_by_horizon": horizon,
            "speedup_vs_plain": (args.steps / total) / plain["value"] if plain else None,
            "roofline": roofline_of(timings, cells, ckpt, nt, args.steps, total)}
@@ -320,11 +344,20 @@ def run_c3(args):
     log(f"[c3] {B} states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
     from fkadj import autotune
+    nt = int(round(T / W.DT_MS))
+    # spare HBM keeps the same number of last trajectory segments of every instance: what is left
+    # after the batch's own buffers, the FP64 instance and 16 GB
+    k0 = int(np.argmin(sizes))
+    free0, _ = fkadj.mem_info()
+    own = sum(footprint(n, n, nt, 50, fkadj.F32, autotune.lookup(n)[1]) for n in sizes)
+    spare = 0 if args.no_keep else free0 - own - footprint(sizes[k0], sizes[k0], nt, 50, fkadj.F64) - (16 << 30)
+    kept, keeps = kept_budgets([(n, n) for n in sizes], nt, 50, spare)
+    log(f"[c3] keeping {kept} of {(nt + 49) // 50} segments of every instance ({sum(keeps) / 1e9:.0f} GB)")
     insts = []
-    for m, y in zip(masks, states):
+    for m, y, keep in zip(masks, states, keeps):
         _, at = autotune.lookup(m.shape[0])        # adjoint tile tuned per grid size
         ft = 0                                      # forward tile: built-in rule by launch size
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at)
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at, keep_bytes=keep)
         inst.set_state(y)
         insts.append(inst)
     for w in range(args.warmup):
@@ -354,7 +387,6 @@ def run_c3(args):
     total = time.perf_counter() - t_start
     plan.close()
     graph_same = all(np.array_equal(a[0], b[0]) and a[1] == b[1] for a, b in zip(res, res_stream))
-    k0 = int(np.argmin(sizes))
     g_alone = insts[k0].grad(A[k0])
     same = bool(np.array_equal(g_alone[0], res[k0][0]))
     inst64 = W.make_instance(masks[k0], T, K_amp, precision=fkadj.F64, ckpt=50)
@@ -362,13 +394,12 @@ def run_c3(args):
     g64 = inst64.grad(A[k0])
     inst64.close()
     cells = sum(i.nrows * ((i.ncols + 63) // 64 * 64) for i in insts)
-    nt = int(round(T / W.DT_MS))
     out = {"metric": "gradient evals/s, C3 batched uneven grids", "value": B * args.steps / total, "unit": "evals/s",
            "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": float(np.mean(step_ms)),
            "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
            "config": {"workload": f"C3: {B} instances, grids {sorted(set(sizes))} (counts "
                                   f"{[sizes.count(n) for n in sorted(set(sizes))]}), T={T:g} ms, {K_amp} amplitudes, K=50",
-                      "cells": cells, "nt": nt},
+                      "cells": cells, "nt": nt, "kept_segments": kept},
            "alone_equals_batched": same, "graph_equals_stream": graph_same,
            "launch": "captured HIP graph of the whole evaluation (replayed per step)",
            "graph_capture_s": capture_s,
@@ -377,7 +408,7 @@ def run_c3(args):
            "rel_err_vs_f64_smallest": {"grad": float(np.linalg.norm(res[k0][0] - g64[0]) / np.linalg.norm(g64[0])),
                                        "J": float(abs(res[k0][1] / g64[1] - 1))},
            "kernels_ms": {"fwd": tm["fwd_kernel_ms"], "adj": tm["adj_kernel_ms"]},
-           "eval_GBs_algorithmic": algorithmic_bytes_per_cell_step(50) * cells * nt * args.steps / total / 1e9}
+           "eval_GBs_algorithmic": 4 * cells * (7 * tm["fwd_launches"] + 13 * tm["adj_launches"]) * args.steps / total / 1e9}
     print(json.dumps(out), flush=True)
     for inst in insts:
         inst.close()
@@ -466,7 +497,7 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--latency-evals", type=int, default=4)
     ap.add_argument("--no-keep", action="store_true",
-                    help="C2: recompute every segment (no trajectory segments kept in spare HBM)")
+                    help="C2, C3, C5: recompute every segment (no trajectory segments kept in spare HBM)")
     ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
                     help="c2 (default): 16 x C2 batched; c1: 128^2 FP64 with the CPU oracle; c3: 64 uneven "
                          "grids; c4: served Poisson stream; c5: 2048^2, 25 000 steps")
@@ -507,18 +538,14 @@ def main():
     fwd_tile = 0
     # spare HBM keeps the last trajectory segments of every instance (no recompute for them): what
     # is left after the batch's own buffers, a single-instance FP32 and an FP64 instance, and 16 GB
-    pr = fkadj.Problem()
-    pr.nrows = pr.ncols = N
-    pr.nt, pr.ckpt, pr.precision, pr.adj_tile = nt, ckpt, fkadj.F32, adj_tile
-    fp32 = fkadj.footprint_bytes(pr)
-    pr.precision = fkadj.F64
-    fp64 = fkadj.footprint_bytes(pr)
+    fp32 = footprint(N, N, nt, ckpt, fkadj.F32, adj_tile)
+    fp64 = footprint(N, N, nt, ckpt, fkadj.F64)
     free0, total0 = fkadj.mem_info()
-    keep = 0 if args.no_keep else max(0, (free0 - (B + 1) * fp32 - fp64 - (16 << 30)) // B)
-    seg_bytes = (2 * ckpt + 1) * 3 * 4 * N * ((N + 63) // 64 * 64)
-    kept = min(keep // seg_bytes, (nt + ckpt - 1) // ckpt)
+    spare = 0 if args.no_keep else free0 - (B + 1) * fp32 - fp64 - (16 << 30)
+    kept, keeps = kept_budgets([(N, N)] * B, nt, ckpt, spare)
+    keep = keeps[0]
     log(f"[bench] {free0 / 1e9:.0f} of {total0 / 1e9:.0f} GB free; keeping {kept} of {(nt + ckpt - 1) // ckpt} "
-        f"segments per instance ({kept * seg_bytes / 1e9:.1f} GB each)")
+        f"segments per instance ({keep / 1e9:.1f} GB each)")
     insts = []
     for m, y in zip(masks, states):
         inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile,

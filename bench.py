@@ -28,10 +28,10 @@ HBM_PEAK_GBS = 8000.0
 # benchmark's launch shapes), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2
 # batch and scaled by the factors a 1 GiB copy kernel gives those counters (x2.0 and x1.0); forward is
 # the mean of forward-sweep and recompute launches
-HBM_BYTES_PER_CELL = {"adj": 75.5, "fwd": 32.8}
+HBM_BYTES_PER_CELL = {"adj": 76.2, "fwd": 32.8}
 # VALU lane-instructions per cell of one launch, same launch shapes and runs (SQ_INSTS_VALU x 64 / cells);
 # the issue peak is one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz (non-packed FP32)
-VALU_LANE_INSTR_PER_CELL = {"adj": 831.0, "fwd": 526.0}
+VALU_LANE_INSTR_PER_CELL = {"adj": 820.0, "fwd": 512.0}
 VALU_PEAK_G_WAVE_INSTR_S = 256 * 2.4
 
 

@@ -202,6 +202,35 @@ def test_graph_plan_replays_equal_eager():
         inst.close()
 
 
+def test_graph_plan_with_kept_segments():
+    """A plan over instances keeping all, some and none of their segments replays the eager batched
+    results, which equal each instance alone without kept segments, bit for bit."""
+    _print_case("graph_plan_kept", sizes="(64,64),(96,128),(40,192)")
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K, ck = 60, 0.02, 6, 20
+    insts, plain, Es = [], [], []
+    for k, ((H, W), keep) in enumerate(zip([(64, 64), (96, 128), (40, 192)], (1 << 40, 1, 0))):
+        case = disc_case(H, W, 6 + 2 * k, 60 + k)
+        seg = (2 * ck + 1) * 3 * 4 * H * ((W + 63) // 64 * 64)
+        for kb, out in ((keep * seg if keep < 1 << 40 else keep, insts), (0, plain)):
+            inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                                  alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32,
+                                  ckpt=ck, keep_bytes=kb)
+            inst.set_state(case.y0)
+            out.append(inst)
+        Es.append(np.random.default_rng(k).uniform(0, 10, K))
+    alone = [inst.grad(E) for inst, E in zip(plain, Es)]
+    eager = fkadj.batch_grad(insts, Es)
+    with fkadj.Plan(insts) as plan:
+        for rep in range(3):
+            got = plan.grad(Es)
+            for a, g, e in zip(alone, got, eager):
+                assert np.array_equal(a[0], g[0]) and a[1] == g[1] and np.array_equal(a[0], e[0]), rep
+            insts[0].grad(Es[0])                                # eager work between replays
+    for inst in insts + plain:
+        inst.close()
+
+
 def test_forward_snapshots_match_oracle(crop):
     _print_case("forward_snapshots", shape=crop.mask.shape)
     import oracle as O

@@ -13,7 +13,7 @@ def _print_case(name, **kw):
     print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
 
 
-def _instances(specs, lazy=True):
+def _instances(specs, lazy=True, keep_bytes=0):
     """specs: (H, W, nt, n_amp, ckpt, seed) -> FP32 piecewise instances with disc holes."""
     m = fkadj.reference_model(h=0.025)
     out = []
@@ -21,7 +21,7 @@ def _instances(specs, lazy=True):
         case = disc_case(H, W, 5 + seed % 4, seed)
         inst = fkadj.Instance(case.ai, H, W, dt=0.02, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
                               alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=ck,
-                              lazy_workspace=lazy)
+                              lazy_workspace=lazy, keep_bytes=keep_bytes)
         inst.set_state(case.y0)
         A = np.random.default_rng(seed).uniform(0, 10, K)
         out.append((inst, A))
@@ -160,5 +160,22 @@ def test_knot_mode_requests_equal_alone():
         served = [srv.poll(i) for i in ids]
     for a, b, c in zip(alone, batched, served):
         assert np.array_equal(a[0], b[0]) and np.array_equal(a[0], c[0]) and a[1] == b[1] == c[1]
+    for inst, _ in pairs:
+        inst.close()
+
+
+def test_requests_on_instances_with_kept_segments_equal_alone():
+    """Instances keeping trajectory segments for their own evaluations: a served request (whose
+    buffers keep none) gives what fkadj_grad on the instance gives."""
+    _print_case("server_kept", n=3)
+    pairs = _instances(SPECS[:3], lazy=False, keep_bytes=1 << 40)
+    alone = [inst.grad(A) for inst, A in pairs]
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=8) as srv:
+        ids = [srv.submit(inst, A) for inst, A in pairs]
+        while srv.pending():
+            srv.run(0)
+        res = [srv.poll(i) for i in ids]
+    for a, r in zip(alone, res):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
     for inst, _ in pairs:
         inst.close()

@@ -709,9 +709,9 @@ void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStr
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
 }
 
-void group_forward2(const FusedGroupDev *g, int step, hipStream_t s)
+void group_forward2(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
-    const Cursors c = uniform_cursors((int)g->members.size(), MODE_FWD, step, 0);
+    const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
     fused_forward2_launch(g->d_slots, g->fwd2, c, s);
 }
 
@@ -728,13 +728,18 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     for (auto *C : g->members)
         HIPCHK(hipMemcpyAsync(C->I->state(C->d_ck, 0), C->I->d_y0, C->I->st_bytes, hipMemcpyDeviceToDevice, s));
     // steps of the kept segments (from keep0 on) store their states and slopes as they go; keep0*K
-    // is a checkpoint, and even where two steps run per launch (K even)
+    // is a checkpoint, and even where two steps run per launch (K even, so two steps never straddle
+    // a segment boundary)
     const int kept_from = std::min(g->keep0 * K, nt);
-    if (g->depth2)
-        for (int i = 0; i < kept_from; i += 2) smp.fwd_launch(s, [&] { group_forward2(g, i, s); });
-    else
+    if (g->depth2) {
+        for (int i = 0; i < kept_from; i += 2) smp.fwd_launch(s, [&] { group_forward2(g, MODE_FWD, i, 0, s); });
+        for (int i = kept_from; i < nt; i += 2)
+            smp.fwd_launch(s, [&] { group_forward2(g, MODE_KEEP, i, i / K * K, s); });
+    } else {
         for (int i = 0; i < kept_from; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_FWD, i, 0, s); });
-    for (int i = kept_from; i < nt; i++) smp.fwd_launch(s, [&] { group_forward(g, MODE_KEEP, i, i / K * K, s); });
+        for (int i = kept_from; i < nt; i++)
+            smp.fwd_launch(s, [&] { group_forward(g, MODE_KEEP, i, i / K * K, s); });
+    }
     HIPCHK(hipGetLastError());
     for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step

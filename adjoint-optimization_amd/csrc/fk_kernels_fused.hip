@@ -154,8 +154,9 @@ __device__ inline void load_region(const FusedSlot &S, const float *__restrict__
 
 // ------------------------------------------------------------------------------ forward
 // WPE: the occupancy asked of the register allocator (waves per SIMD; 1 = no request).  D: time
-// steps per launch (D > 1 only for the forward sweep, MODE_FWD): the step-(st+1) state of every cell
-// within distance 4(D-1-st) of the tile is formed in the launch, so D steps read and write HBM once.
+// steps per launch (D > 1 only for the forward sweep, MODE_FWD and MODE_KEEP): the step-(st+1) state
+// of every cell within distance 4(D-1-st) of the tile is formed in the launch, so D steps read and
+// write HBM once (MODE_KEEP also stores the tile's intermediate states and every step's slope).
 template <int TY, int NT, int WPE, int D = 1>
 __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
@@ -167,7 +168,7 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
 
     const TileRef t = launch_tile<TY>(slots, map);
     const SlotCursor sc = cur.c[t.slot];
-    if (sc.mode != MODE_FWD && (D > 1 || (sc.mode != MODE_REC && sc.mode != MODE_RHS && sc.mode != MODE_KEEP)))
+    if (sc.mode != MODE_FWD && sc.mode != MODE_KEEP && (D > 1 || (sc.mode != MODE_REC && sc.mode != MODE_RHS)))
         return;   // uniform per block
     const int mode = sc.mode, step = sc.step, seg_a = sc.seg_a;
     const FusedSlot &S = slots[t.slot];
@@ -175,8 +176,9 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
     const size_t st = (size_t)3 * S.n;
     const float *yin;
     float *yout;
-    float *kout = nullptr;      // stage-1 slopes of the tile (MODE_REC, MODE_RHS)
+    float *kout = nullptr;      // stage-1 slopes of the tile (MODE_REC, MODE_KEEP: one per step; MODE_RHS)
     float *kout2 = nullptr;     // second copy of the first slope of a segment (MODE_REC, j = 0)
+    float *ymid = nullptr;      // MODE_KEEP with D > 1: the tile's states after the inner steps
     float E1, E23, E4;
     if (mode == MODE_FWD) {
         // between checkpoints the sweep alternates between the two ring states, launch by launch
@@ -190,12 +192,14 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
         // buffers; the first slope of each recomputed segment is also kept by segment index mod 3
         // for the adjoint step that ends at it (the last one of the previous segment), which reads
         // a kept segment's first slope from that segment's own buffer
-        const int j = step - seg_a, segi = seg_a / S.K, buf = seg_buf(S, segi), nx = step + 1;
+        // (D > 1: the D steps stay inside one segment, the host keeps the step and K multiples of D)
+        const int j = step - seg_a, segi = seg_a / S.K, buf = seg_buf(S, segi), nx = step + D;
         float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
         yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
         yout = (mode == MODE_KEEP && (nx % S.K == 0 || nx == S.nt)) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
-                                                                   : segb + (size_t)(j + 1) * st;
+                                                                   : segb + (size_t)(j + D) * st;
         kout = S.kseg + ((size_t)buf * S.K + j) * st;
+        if (D > 1) ymid = segb + (size_t)(j + 1) * st;
         if (mode == MODE_REC && j == 0 && segi < S.keep0) kout2 = S.kfirst + (size_t)(segi % 3) * st;
     } else {
         yin = S.ck + (size_t)(S.nck - 1) * st;
@@ -248,15 +252,16 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
                     rhs_lds<RY, RX>(m, sY, lw[sl], ly[sl], lx[sl], E, f[sl], yc);
                 }
             }
-            if (D == 1 && k == 1 && kout) {
+            if (k == 1 && kout) {
                 // stage-1 slope f(y_j) of the tile, read back by the adjoint step
+                float *ko = kout + (size_t)ss * st;
 #pragma unroll
                 for (int sl = 0; sl < G::TS; sl++) {
                     int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
                     if (gy >= S.rows) continue;
                     int kk = gy * S.pitch + gx;
 #pragma unroll
-                    for (int v = 0; v < 3; v++) st_at(uniform_ptr(gmem(kout) + (size_t)v * S.n), kk, f[sl][v]);
+                    for (int v = 0; v < 3; v++) st_at(uniform_ptr(gmem(ko) + (size_t)v * S.n), kk, f[sl][v]);
                     if (kout2)
 #pragma unroll
                         for (int v = 0; v < 3; v++) st_at(uniform_ptr(gmem(kout2) + (size_t)v * S.n), kk, f[sl][v]);
@@ -290,6 +295,19 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
                     for (int v = 0; v < 3; v++) {
                         y[sl][v] = y[sl][v] + (acc[sl][v] + b4 * f[sl][v]);
                         sY[v][ly[sl]][lx[sl]] = y[sl][v];
+                    }
+                }
+                if (ymid) {
+                    // the tile's state after this inner step, for the adjoint (kept segments)
+                    float *ym = ymid + (size_t)ss * st;
+#pragma unroll
+                    for (int sl = 0; sl < G::TS; sl++) {
+                        int gy = t.y0 + ly[sl] - G::H, gx = t.x0 + lx[sl] - G::H;
+                        if (gy >= S.rows) continue;
+                        int kk = gy * S.pitch + gx;
+#pragma unroll
+                        for (int v = 0; v < 3; v++)
+                            st_at(uniform_ptr(gmem(ym) + (size_t)v * S.n), kk, (cc[sl] & C_ACTIVE) ? y[sl][v] : 0.0f);
                     }
                 }
                 __syncthreads();
@@ -546,7 +564,7 @@ void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
     }
 }
 
-// Two forward-sweep steps per launch (MODE_FWD only) on 64x8 tiles of 512 threads with an 8-cell
+// Two forward-sweep steps per launch (MODE_FWD and MODE_KEEP) on 64x8 tiles of 512 threads with an 8-cell
 // halo: for launches too small to fill the GPU, where a launch costs more than its arithmetic.
 void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream)
 {

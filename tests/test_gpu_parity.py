@@ -458,7 +458,7 @@ def test_kept_segments_change_nothing():
     from none to all segments, gives the same gradient and J bit for bit, alone and batched with
     members keeping different numbers of segments; the evaluation launches nt forward steps plus the
     recompute of the segments not kept.  Knot-sampled field; nt = 62 with K = 20 (a short last
-    segment, two forward steps per launch before the kept segments) and nt = 61 (one step per
+    segment, two forward-sweep steps per launch, kept ones included) and nt = 61 (one step per
     launch)."""
     m = fkadj.reference_model(h=0.025)
     dt, K = 0.02, 20
@@ -487,7 +487,7 @@ def test_kept_segments_change_nothing():
                     fkadj.set_profiling(False)
                 assert np.array_equal(g[0], base[k][0]) and g[1] == base[k][1], (nt, k, kept)
                 recomputed = max(0, min((nseg - kept) * K, nt))
-                sweep = recomputed // 2 + (nt - recomputed) if nt % 2 == 0 else nt
+                sweep = nt // 2 if nt % 2 == 0 else nt
                 assert t["fwd_launches"] == sweep + recomputed and t["adj_launches"] == nt, (nt, k, kept, t)
         # batched: members keeping 1 and all segments, and none and 2
         for pair in (((0, 1), (1, nseg)), ((0, 0), (1, 2))):

@@ -184,6 +184,27 @@ int main(void)
     CALL(fkadj_destroy(f32a));
     CALL(fkadj_destroy(f64));
 
+    /* 3b. kept trajectory segments (a budget for 3 of the 5 segments) change no result; the footprint
+     *     counts them and fits the free device memory */
+    {
+        int64_t fr = 0, tot = 0, plain_bytes, kept_bytes;
+        fkadj_instance *f32k = NULL;
+        double gk[NAMP];
+        const int64_t seg = (int64_t)(2 * 8 + 1) * 3 * 4 * NR * 128;   /* K = 8, pitch 128 */
+        make_problem(&p, FKADJ_F32, 0.0);
+        plain_bytes = fkadj_footprint_bytes(&p);
+        p.keep_bytes = 3 * seg + seg / 2;
+        kept_bytes = fkadj_footprint_bytes(&p);
+        CHECK(kept_bytes - plain_bytes == 3 * seg, "footprint of 3 kept segments %lld", (long long)(kept_bytes - plain_bytes));
+        CALL(fkadj_mem_info(&fr, &tot));
+        CHECK(0 < fr && fr <= tot && kept_bytes < fr, "free %lld of %lld bytes", (long long)fr, (long long)tot);
+        CALL(fkadj_create(&p, &f32k));
+        CALL(fkadj_set_state(f32k, y0v));
+        CALL(fkadj_grad(f32k, A, gk, NULL, NULL, NULL));
+        CHECK(memcmp(gk, g32, sizeof g32) == 0, "kept segments change the gradient");
+        CALL(fkadj_destroy(f32k));
+    }
+
     /* 4. forward drop-in against a forward run on a handle */
     {
         static double ys[2 * 3 * NR * NC], yf[3 * NR * NC], snaps[2 * 3 * NR * NC], fin[3 * NR * NC];

@@ -141,8 +141,9 @@ def test_batch_equals_alone():
 def test_large_instance_one_segment_buffer():
     """An instance of 2M cells or more keeps one segment state / slope buffer (its backward pass never
     overlaps a recompute with an adjoint): several segments through that one buffer give the
-    single-segment result bit for bit, FP32 stays within 1e-4 of FP64, and a served request holds
-    less than the instance's footprint."""
+    single-segment result bit for bit, and so do two of five segments kept (one stream, one step per
+    launch); FP32 stays within 1e-4 of FP64, and a served request holds less than the instance's
+    footprint."""
     H, W = 1024, 2048
     _print_case("large_one_buffer", shape=(H, W))
     case = disc_case(H, W, 40, 21)
@@ -150,15 +151,19 @@ def test_large_instance_one_segment_buffer():
     nt, dt, K = 24, 0.02, 4
     A = np.random.default_rng(21).uniform(0, 10, K)
     out = {}
-    for prec, ck in ((fkadj.F32, 5), (fkadj.F32, 24), (fkadj.F64, 6)):
+    seg5 = (2 * 5 + 1) * 3 * 4 * H * W
+    for prec, ck, keep in ((fkadj.F32, 5, 0), (fkadj.F32, 24, 0), (fkadj.F32, 5, 2 * seg5), (fkadj.F64, 6, 0)):
         with fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
-                            alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=prec, ckpt=ck) as inst:
+                            alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=prec, ckpt=ck,
+                            keep_bytes=keep) as inst:
             inst.set_state(case.y0)
-            out[(prec, ck)] = inst.grad(A)
+            out[(prec, ck, keep)] = inst.grad(A)
             if prec == fkadj.F32:
                 assert inst.request_bytes < fkadj.footprint_bytes(inst.problem)
-    a, b, r = out[(fkadj.F32, 5)], out[(fkadj.F32, 24)], out[(fkadj.F64, 6)]
+    a, b, r = out[(fkadj.F32, 5, 0)], out[(fkadj.F32, 24, 0)], out[(fkadj.F64, 6, 0)]
+    kept = out[(fkadj.F32, 5, 2 * seg5)]
     assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    assert np.array_equal(a[0], kept[0]) and a[1] == kept[1]
     assert relerr(a[0], r[0]) < 1e-4 and abs(a[1] - r[1]) < 1e-5 * abs(r[1])
 
 

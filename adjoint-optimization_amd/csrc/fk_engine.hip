@@ -18,7 +18,9 @@
 #include <map>
 #include <mutex>
 #include <new>
+#include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/fkadj.h"
@@ -190,6 +192,9 @@ struct FusedGroupDev {
     // segments from keep0 on are kept by every member: the forward sweep fills them and the
     // backward pass does not recompute them
     int keep0 = 0;
+    // a group of a split batch runs on a batch lane stream (from a host thread of its own), forked
+    // from and joined to the caller's stream through ev_lane
+    hipEvent_t ev_lane = nullptr;
 };
 
 namespace {
@@ -645,7 +650,9 @@ void group_free(FusedGroupDev *g)
         if (g->ev_rec[p]) (void)hipEventDestroy(g->ev_rec[p]);
         if (g->ev_adj[p]) (void)hipEventDestroy(g->ev_adj[p]);
     }
+    if (g->ev_lane) (void)hipEventDestroy(g->ev_lane);
     if (g->side) (void)hipStreamDestroy(g->side);
+
     delete g;
 }
 
@@ -686,13 +693,78 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
     }
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
-    ok = ok && hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) == hipSuccess;
+    ok = ok && hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) == hipSuccess &&
+         hipEventCreateWithFlags(&g->ev_lane, hipEventDisableTiming) == hipSuccess;
     for (int p = 0; p < 2 && ok; p++)
         ok = hipEventCreateWithFlags(&g->ev_rec[p], hipEventDisableTiming) == hipSuccess &&
              hipEventCreateWithFlags(&g->ev_adj[p], hipEventDisableTiming) == hipSuccess;
     ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) { group_free(g); return FKADJ_ERR_NOMEM; }
     *out = g;
+    return FKADJ_OK;
+}
+
+// Groups a batch call of 2M cells or more is split into (fkadj_set_batch_groups).
+std::atomic<int> g_batch_groups{2};
+
+// Streams of the groups of split batch calls, per device: created back to back, once, so that they
+// sit on different hardware queues (a stream gets one of the device's few queues when it is created,
+// and two streams on one queue run one after the other).
+constexpr int kLanes = 4;
+std::mutex g_lanes_mu;
+std::map<int, std::vector<hipStream_t>> g_lanes;
+
+int batch_lanes(std::vector<hipStream_t> &out)
+{
+    int dev = 0;
+    HIPCHK(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lk(g_lanes_mu);
+    auto &v = g_lanes[dev];
+    while (v.size() < (size_t)kLanes) {
+        hipStream_t st = nullptr;
+        HIPCHK(hipStreamCreateWithFlags(&st, hipStreamNonBlocking));
+        v.push_back(st);
+    }
+    out = v;
+    return FKADJ_OK;
+}
+
+void groups_free(std::vector<FusedGroupDev *> &gs)
+{
+    for (auto *g : gs) group_free(g);
+    gs.clear();
+}
+
+// The members of a batch call as groups, each run from its own host thread on its own stream: a
+// batch of 2M cells or more (its backward pass runs on one stream) is cut into up to
+// g_batch_groups contiguous runs of members of about equal cells; a smaller one stays one group.
+int groups_build(const std::vector<EvalCtx *> &members, hipStream_t s, std::vector<FusedGroupDev *> &out)
+{
+    out.clear();
+    auto cells_of = [](const EvalCtx *C) { return (int64_t)C->I->rows * C->I->pitch; };
+    int64_t total = 0;
+    for (auto *C : members) total += cells_of(C);
+    const int k = total >= kTwoStreamCells ? std::max(1, std::min(g_batch_groups.load(), (int)members.size())) : 1;
+    std::vector<std::vector<EvalCtx *>> parts(k);
+    size_t q = 0;
+    int64_t acc = 0;
+    for (int j = 0; j < k; j++) {
+        const int64_t share = total * (j + 1) / k;
+        const size_t last = members.size() - (size_t)(k - 1 - j);   // one member left for each later part
+        while (q < last) {
+            const int64_t c = cells_of(members[q]);
+            if (!parts[j].empty() && acc + c / 2 > share) break;   // its larger half lies beyond this share
+            parts[j].push_back(members[q++]);
+            acc += c;
+        }
+    }
+    for (; q < members.size(); q++) parts[k - 1].push_back(members[q]);
+    for (auto &part : parts) {
+        FusedGroupDev *g = nullptr;
+        const int rc = group_build(part, s, &g);
+        if (rc) { groups_free(out); return rc; }
+        out.push_back(g);
+    }
     return FKADJ_OK;
 }
 
@@ -778,17 +850,50 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     return FKADJ_OK;
 }
 
-void record_timing(Sampler &smp, hipEvent_t e0, hipEvent_t e1, int64_t cell_steps)
+// Timing of the last evaluation from the samplers of its groups (launch counts summed, sampled
+// durations averaged over all groups' samples).
+void record_timing(const std::vector<Sampler *> &smps, hipEvent_t e0, hipEvent_t e1, int64_t cell_steps)
 {
     float ms = 0.f;
     (void)hipEventElapsedTime(&ms, e0, e1);
+    std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd, adj;
+    int64_t nf = 0, na = 0;
+    for (auto *smp : smps) {
+        fwd.insert(fwd.end(), smp->fwd.begin(), smp->fwd.end());
+        adj.insert(adj.end(), smp->adj.begin(), smp->adj.end());
+        nf += smp->nf;
+        na += smp->na;
+    }
+    const double fms = Sampler::mean(fwd), ams = Sampler::mean(adj);
     std::lock_guard<std::mutex> lk(g_timing_mu);
     g_timing.eval_ms = ms;
-    g_timing.fwd_kernel_ms = Sampler::mean(smp.fwd);
-    g_timing.adj_kernel_ms = Sampler::mean(smp.adj);
-    g_timing.fwd_launches = smp.nf;
-    g_timing.adj_launches = smp.na;
+    g_timing.fwd_kernel_ms = fms;
+    g_timing.adj_kernel_ms = ams;
+    g_timing.fwd_launches = nf;
+    g_timing.adj_launches = na;
     g_timing.cell_steps = cell_steps;
+}
+void record_timing(Sampler &smp, hipEvent_t e0, hipEvent_t e1, int64_t cell_steps)
+{
+    record_timing(std::vector<Sampler *>{&smp}, e0, e1, cell_steps);
+}
+
+// run_fused of one group, for a host thread of its own: nothing escapes the thread, and a device
+// error keeps the thread's message
+struct GroupRun {
+    int rc = FKADJ_OK;
+    char err[sizeof g_errmsg] = {0};
+};
+void run_group(FusedGroupDev *g, hipStream_t st, Sampler *smp, GroupRun *out)
+{
+    try {
+        out->rc = run_fused(g, st, *smp);
+    } catch (const std::bad_alloc &) {
+        out->rc = FKADJ_ERR_NOMEM;
+    } catch (...) {
+        out->rc = FKADJ_ERR_DEVICE;
+    }
+    if (out->rc == FKADJ_ERR_DEVICE) memcpy(out->err, g_errmsg, sizeof out->err);
 }
 
 int fetch_and_assemble(EvalCtx *C, hipStream_t s, const double *E, double *grad, double *Lc, double *L0,
@@ -1306,6 +1411,12 @@ int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, con
     FKADJ_EXIT
 }
 
+int fkadj_set_batch_groups(int n)
+{
+    if (n < 1 || n > 8) return FKADJ_ERR_ARG;
+    return g_batch_groups.exchange(n);
+}
+
 int fkadj_set_profiling(int on)
 {
     g_profiling.store(on ? 1 : 0);
@@ -1342,11 +1453,48 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
         members.push_back(insts[k]->ctx);
         CHK(build_tables(insts[k], insts[k]->ctx, E[k], s));
     }
-    FusedGroupDev *g = nullptr;
-    CHK(group_build(members, s, &g));
-    Sampler smp(g_profiling.load() != 0);
-    hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
-    int rc = hipEventRecord(e0, s) == hipSuccess ? run_fused(g, s, smp) : FKADJ_ERR_DEVICE;
+    std::vector<FusedGroupDev *> gs;
+    CHK(groups_build(members, s, gs));
+    const size_t ng = gs.size();
+    std::vector<std::unique_ptr<Sampler>> smp;
+    std::vector<Sampler *> smps;
+    for (size_t j = 0; j < ng; j++) {
+        smp.emplace_back(new Sampler(g_profiling.load() != 0));
+        smps.push_back(smp.back().get());
+    }
+    hipEvent_t e0 = smp[0]->pool.get(), e1 = smp[0]->pool.get();
+    int rc = hipEventRecord(e0, s) == hipSuccess ? FKADJ_OK : FKADJ_ERR_DEVICE;
+    // a split batch: every group starts on a lane stream of its own when the caller's stream gets
+    // here, each issued from a host thread of its own (one thread issuing every group's launches
+    // would be the bottleneck), and the caller's stream waits for all of them at the end
+    std::vector<hipStream_t> lane(ng, s);
+    if (ng > 1 && !rc) {
+        std::vector<hipStream_t> lanes;
+        rc = batch_lanes(lanes);
+        for (size_t j = 0; j < ng && !rc; j++) {
+            lane[j] = lanes[j % lanes.size()];
+            if (hipEventRecord(gs[j]->ev_lane, s) != hipSuccess || hipStreamWaitEvent(lane[j], gs[j]->ev_lane, 0) != hipSuccess)
+                rc = FKADJ_ERR_DEVICE;
+        }
+    }
+    if (!rc) {
+        std::vector<GroupRun> runs(ng);
+        std::vector<std::thread> threads;
+        try {
+            for (size_t j = 1; j < ng; j++) threads.emplace_back(run_group, gs[j], lane[j], smps[j], &runs[j]);
+        } catch (...) {
+            runs[0].rc = FKADJ_ERR_NOMEM;   // a group without its thread: the call fails after the others
+        }
+        if (!runs[0].rc) run_group(gs[0], lane[0], smps[0], &runs[0]);
+        for (auto &t : threads) t.join();
+        for (size_t j = 0; j < ng && !rc; j++) {
+            rc = runs[j].rc;
+            if (rc == FKADJ_ERR_DEVICE) memcpy(g_errmsg, runs[j].err, sizeof g_errmsg);
+        }
+    }
+    for (size_t j = 0; j < ng && ng > 1; j++)
+        if (hipEventRecord(gs[j]->ev_lane, lane[j]) != hipSuccess || hipStreamWaitEvent(s, gs[j]->ev_lane, 0) != hipSuccess)
+            rc = rc ? rc : FKADJ_ERR_DEVICE;
     if (!rc && hipEventRecord(e1, s) != hipSuccess) rc = FKADJ_ERR_DEVICE;
     int64_t cells = 0;
     for (int k = 0; k < count && !rc; k++) {
@@ -1354,9 +1502,9 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
                                 L0 ? L0 + k : nullptr, LE ? LE + k : nullptr);
         cells += (int64_t)insts[k]->rows * insts[k]->pitch * insts[k]->nt;
     }
-    if (!rc) record_timing(smp, e0, e1, cells);
+    if (!rc) record_timing(smps, e0, e1, cells);
     (void)hipStreamSynchronize(s);
-    group_free(g);
+    groups_free(gs);
     return rc;
     FKADJ_EXIT
 }

@@ -105,6 +105,7 @@ def lib():
     L.fkadj_server_poll.argtypes = [vp, C.c_int64, dp, dp, dp, dp]
     L.fkadj_server_pending.argtypes = [vp]
     L.fkadj_set_profiling.argtypes = [C.c_int]
+    L.fkadj_set_batch_groups.argtypes = [C.c_int]
     L.fkadj_last_timing.argtypes = [C.POINTER(Timing)]
     _lib = L
     return L
@@ -147,6 +148,14 @@ def param_string(model: Model | None = None) -> str:
 
 def set_profiling(on: bool):
     lib().fkadj_set_profiling(1 if on else 0)
+
+
+def set_batch_groups(n: int) -> int:
+    """Groups (host threads and streams) a batch call of 2M cells or more runs as; returns the
+    previous setting."""
+    rc = lib().fkadj_set_batch_groups(int(n))
+    check(min(rc, 0), "fkadj_set_batch_groups")
+    return rc
 
 
 def last_timing() -> dict:

@@ -207,9 +207,19 @@ typedef struct fkadj_timing {
     int64_t cell_steps;      /* active+padding cells x steps processed per sweep */
 } fkadj_timing;
 
-/* Enables per-launch HIP events on the next calls (adds small overhead); 0 disables. */
+/* Enables per-launch HIP events on the next calls (adds small overhead); 0 disables.  The events
+ * time each launch on its own stream: the groups of a split batch call (fkadj_set_batch_groups) run
+ * at the same time, so their launches overlap; set 1 group to time kernels alone. */
 int fkadj_set_profiling(int on);
 int fkadj_last_timing(fkadj_timing *t);
+/* fkadj_batch_grad calls over 2M cells or more run their members as up to n groups (contiguous
+ * runs of about equal cells), each issued from a host thread of its own on one of the library's
+ * batch streams (created once per device, back to back, so that they sit on different hardware
+ * queues), so one group's kernels fill the GPU while another's drain (16 x 512^2: 4-7% faster with
+ * 2; 3 and 4 measured slower).  Results do not depend on n; plans always run one group; concurrent
+ * split calls from several host threads share the batch streams.  Process-wide, 1..8, default 2;
+ * returns the previous value.  No reference counterpart. */
+int fkadj_set_batch_groups(int n);
 
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching gradient server on one GPU (one host thread drives it).  A request is

@@ -207,6 +207,41 @@ def test_graph_plan_replays_equal_eager():
         inst.close()
 
 
+def test_split_batches_equal_alone():
+    """A batch call of 2M cells or more runs as up to fkadj_set_batch_groups groups, each from its own
+    host thread on its own stream: 1, 2 and 3 groups give each instance's own result bit for bit (one
+    member keeps all of its segments, one some); a plan made meanwhile runs one group and agrees."""
+    _print_case("split_batch", sizes="(1024,768),(512,1024),(1024,1024)")
+    m = fkadj.reference_model(h=0.025)
+    nt, dt, K, ck = 20, 0.02, 4, 10
+    insts, Es = [], []
+    for k, ((H, W), keep) in enumerate(zip([(1024, 768), (512, 1024), (1024, 1024)], (1 << 40, 0, 1))):
+        case = disc_case(H, W, 20 + 4 * k, 80 + k)
+        seg = (2 * ck + 1) * 3 * 4 * H * ((W + 63) // 64 * 64)
+        inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
+                              alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=ck,
+                              keep_bytes=keep if keep != 1 else seg)
+        inst.set_state(case.y0)
+        insts.append(inst)
+        Es.append(np.random.default_rng(k).uniform(0, 10, K))
+    alone = [inst.grad(E) for inst, E in zip(insts, Es)]
+    prev = fkadj.set_batch_groups(1)
+    try:
+        for n in (1, 2, 3):
+            fkadj.set_batch_groups(n)
+            for rep in range(2):
+                got = fkadj.batch_grad(insts, Es)
+                for a, g in zip(alone, got):
+                    assert np.array_equal(a[0], g[0]) and a[1] == g[1] and a[2] == g[2], (n, rep)
+        with fkadj.Plan(insts) as plan:
+            for a, g in zip(alone, plan.grad(Es)):
+                assert np.array_equal(a[0], g[0]) and a[1] == g[1]
+    finally:
+        fkadj.set_batch_groups(prev)
+    for inst in insts:
+        inst.close()
+
+
 def test_graph_plan_with_kept_segments():
     """A plan over instances keeping all, some and none of their segments replays the eager batched
     results, which equal each instance alone without kept segments, bit for bit."""

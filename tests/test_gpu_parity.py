@@ -498,19 +498,19 @@ def test_kept_segments_change_nothing():
     from none to all segments, gives the same gradient and J bit for bit, alone and batched with
     members keeping different numbers of segments; the evaluation launches nt forward steps plus the
     recompute of the segments not kept.  Knot-sampled field; nt = 62 with K = 20 (a short last
-    segment, two forward-sweep steps per launch, kept ones included) and nt = 61 (one step per
-    launch)."""
+    segment, two forward-sweep steps per launch, kept ones included), nt = 61 (one step per launch)
+    and nt = 30 with K = 40 (one segment, shorter than K)."""
     m = fkadj.reference_model(h=0.025)
-    dt, K = 0.02, 20
+    dt = 0.02
     shapes = [(64, 64), (96, 128)]
-    for nt in (62, 61):
+    for nt, K in ((62, 20), (61, 20), (30, 40)):
         _print_case("kept_segments", sizes="(64,64),(96,128)", nt=nt, K=K)
         nseg = (nt + K - 1) // K
         base, insts = {}, {}
         for k, (H, W) in enumerate(shapes):
             case = disc_case(H, W, 6 + 2 * k, 40 + k)
             seg = (2 * K + 1) * 3 * 4 * H * ((W + 63) // 64 * 64)
-            for kept in (0, 1, 2, nseg):
+            for kept in sorted({0, 1, 2, nseg}):
                 inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, alpha=2e-3, gamma=(1, 0, 0),
                                       s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=K,
                                       keep_bytes=kept * seg + (seg // 2 if kept < nseg else 1 << 40))
@@ -518,7 +518,7 @@ def test_kept_segments_change_nothing():
                 insts[k, kept] = inst
             E = _Et(nt, k)
             base[k] = insts[k, 0].grad(E)
-            for kept in (1, 2, nseg):
+            for kept in sorted({1, 2, nseg}):
                 fkadj.set_profiling(True)
                 try:
                     g = insts[k, kept].grad(E)

@@ -736,15 +736,16 @@ void groups_free(std::vector<FusedGroupDev *> &gs)
 }
 
 // The members of a batch call as groups, each run from its own host thread on its own stream: a
-// batch of 2M cells or more (its backward pass runs on one stream) is cut into up to
-// g_batch_groups contiguous runs of members of about equal cells; a smaller one stays one group.
-int groups_build(const std::vector<EvalCtx *> &members, hipStream_t s, std::vector<FusedGroupDev *> &out)
+// batch of 2M cells or more (its backward pass runs on one stream) is cut into up to max_groups
+// contiguous runs of members of about equal cells; a smaller one stays one group.
+int groups_build(const std::vector<EvalCtx *> &members, hipStream_t s, int max_groups,
+                 std::vector<FusedGroupDev *> &out)
 {
     out.clear();
     auto cells_of = [](const EvalCtx *C) { return (int64_t)C->I->rows * C->I->pitch; };
     int64_t total = 0;
     for (auto *C : members) total += cells_of(C);
-    const int k = total >= kTwoStreamCells ? std::max(1, std::min(g_batch_groups.load(), (int)members.size())) : 1;
+    const int k = total >= kTwoStreamCells ? std::max(1, std::min(max_groups, (int)members.size())) : 1;
     std::vector<std::vector<EvalCtx *>> parts(k);
     size_t q = 0;
     int64_t acc = 0;
@@ -1453,15 +1454,19 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
         members.push_back(insts[k]->ctx);
         CHK(build_tables(insts[k], insts[k]->ctx, E[k], s));
     }
-    std::vector<FusedGroupDev *> gs;
-    CHK(groups_build(members, s, gs));
-    const size_t ng = gs.size();
+    // one sampler per possible group, made before any device resource so that nothing leaks if
+    // host memory runs out
+    const int max_groups = std::max(1, std::min(g_batch_groups.load(), count));
     std::vector<std::unique_ptr<Sampler>> smp;
     std::vector<Sampler *> smps;
-    for (size_t j = 0; j < ng; j++) {
-        smp.emplace_back(new Sampler(g_profiling.load() != 0));
+    for (int j = 0; j < max_groups; j++) {
+        smp.push_back(std::make_unique<Sampler>(g_profiling.load() != 0));
         smps.push_back(smp.back().get());
     }
+    std::vector<FusedGroupDev *> gs;
+    CHK(groups_build(members, s, max_groups, gs));
+    const size_t ng = gs.size();   // at most max_groups
+    smps.resize(ng);
     hipEvent_t e0 = smp[0]->pool.get(), e1 = smp[0]->pool.get();
     int rc = hipEventRecord(e0, s) == hipSuccess ? FKADJ_OK : FKADJ_ERR_DEVICE;
     // a split batch: every group starts on a lane stream of its own when the caller's stream gets

@@ -123,6 +123,21 @@ def test_footprint_counts_kept_segments():
     assert fkadj.footprint_bytes(p) == with_budget
 
 
+def test_batch_groups_setting():
+    """The split-batch setting takes 1..8 and returns the previous value (no device work)."""
+    prev = fkadj.set_batch_groups(3)
+    try:
+        assert 1 <= prev <= 8
+        assert fkadj.set_batch_groups(1) == 3
+        for bad in (0, 9, -1):
+            with pytest.raises(fkadj.FkadjError) as e:
+                fkadj.set_batch_groups(bad)
+            assert e.value.code == L.ERR_ARG
+        assert fkadj.set_batch_groups(prev) == 1
+    finally:
+        fkadj.set_batch_groups(prev)
+
+
 def test_disc_holes_have_no_isthmus_and_closed_form_ghost():
     m = G.disc_holes(96, 128, 25, 2, 6, seed=7)
     assert not G.isthmus_cells(m).any()

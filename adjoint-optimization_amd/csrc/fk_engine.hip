@@ -1520,11 +1520,14 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
 // A captured HIP graph of one whole evaluation of a fixed group.  The graph holds kernel and
 // device-to-device copy nodes only (no memset nodes); it reads the field tables and the resident
 // states at replay time, so a replay after new tables (or a new state) computes the new gradient.
+// A plan of 2M cells or more holds its members as groups, like a batch call, with one graph per
+// group (each captured alone on the caller's stream); a replay launches the graphs side by side on
+// the batch streams.
 struct fkadj_plan {
     std::vector<fkadj_instance *> insts;
-    FusedGroupDev *g = nullptr;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t exec = nullptr;
+    std::vector<FusedGroupDev *> gs;
+    std::vector<hipGraph_t> graphs;
+    std::vector<hipGraphExec_t> execs;
     hipStream_t stream = nullptr;
 };
 
@@ -1550,9 +1553,9 @@ void plan_free(fkadj_plan *P)
 {
     if (!P) return;
     if (P->stream) (void)hipStreamSynchronize(P->stream);
-    if (P->exec) (void)hipGraphExecDestroy(P->exec);
-    if (P->graph) (void)hipGraphDestroy(P->graph);
-    group_free(P->g);
+    for (auto x : P->execs) (void)hipGraphExecDestroy(x);
+    for (auto g : P->graphs) (void)hipGraphDestroy(g);
+    groups_free(P->gs);
     for (auto *I : P->insts) I->plans--;
     delete P;
 }
@@ -1578,24 +1581,28 @@ int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out)
         if (rc) { plan_free(P); return rc; }
         members.push_back(insts[k]->ctx);
     }
-    int rc = group_build(members, P->stream, &P->g);
+    int rc = groups_build(members, P->stream, std::max(1, std::min(g_batch_groups.load(), count)), P->gs);
     if (rc) { plan_free(P); return rc; }
-    Sampler off(false);
-    if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
-        plan_free(P);
-        return FKADJ_ERR_DEVICE;
-    }
-    rc = run_fused(P->g, P->stream, off);
-    hipGraph_t graph = nullptr;
-    hipError_t e = hipStreamEndCapture(P->stream, &graph);
-    P->graph = graph;
-    if (rc == FKADJ_OK && e != hipSuccess) rc = FKADJ_ERR_DEVICE;
-    if (rc == FKADJ_OK && hipGraphInstantiate(&P->exec, P->graph, nullptr, nullptr, 0) != hipSuccess)
-        rc = FKADJ_ERR_DEVICE;
-    if (rc) {
-        (void)hipGetLastError();
-        plan_free(P);
-        return rc;
+    for (auto *g : P->gs) {
+        Sampler off(false);
+        if (hipStreamBeginCapture(P->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+            plan_free(P);
+            return FKADJ_ERR_DEVICE;
+        }
+        rc = run_fused(g, P->stream, off);
+        hipGraph_t graph = nullptr;
+        hipError_t e = hipStreamEndCapture(P->stream, &graph);
+        if (graph) P->graphs.push_back(graph);
+        hipGraphExec_t exec = nullptr;
+        if (rc == FKADJ_OK && e != hipSuccess) rc = FKADJ_ERR_DEVICE;
+        if (rc == FKADJ_OK && hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0) != hipSuccess)
+            rc = FKADJ_ERR_DEVICE;
+        if (exec) P->execs.push_back(exec);
+        if (rc) {
+            (void)hipGetLastError();
+            plan_free(P);
+            return rc;
+        }
     }
     *out = P;
     return FKADJ_OK;
@@ -1614,7 +1621,23 @@ int fkadj_plan_grad(fkadj_plan *P, const double *const *E, double *const *grad, 
     Sampler smp(false);
     hipEvent_t e0 = smp.pool.get(), e1 = smp.pool.get();
     HIPCHK(hipEventRecord(e0, s));
-    HIPCHK(hipGraphLaunch(P->exec, s));
+    if (P->execs.size() == 1) {
+        HIPCHK(hipGraphLaunch(P->execs[0], s));
+    } else {
+        // each group's graph on a batch stream of its own, forked from and joined to s
+        std::vector<hipStream_t> lanes;
+        CHK(batch_lanes(lanes));
+        const size_t ng = P->execs.size();
+        for (size_t j = 0; j < ng; j++) {   // every fork before any join, so the graphs run together
+            HIPCHK(hipEventRecord(P->gs[j]->ev_lane, s));
+            HIPCHK(hipStreamWaitEvent(lanes[j % lanes.size()], P->gs[j]->ev_lane, 0));
+        }
+        for (size_t j = 0; j < ng; j++) HIPCHK(hipGraphLaunch(P->execs[j], lanes[j % lanes.size()]));
+        for (size_t j = 0; j < ng; j++) {
+            HIPCHK(hipEventRecord(P->gs[j]->ev_lane, lanes[j % lanes.size()]));
+            HIPCHK(hipStreamWaitEvent(s, P->gs[j]->ev_lane, 0));
+        }
+    }
     HIPCHK(hipEventRecord(e1, s));
     int64_t cells = 0;
     for (int k = 0; k < count; k++) {

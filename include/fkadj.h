@@ -216,7 +216,8 @@ int fkadj_last_timing(fkadj_timing *t);
  * runs of about equal cells), each issued from a host thread of its own on one of the library's
  * batch streams (created once per device, back to back, so that they sit on different hardware
  * queues), so one group's kernels fill the GPU while another's drain (16 x 512^2: 4-7% faster with
- * 2; 3 and 4 measured slower).  Results do not depend on n; plans always run one group; concurrent
+ * 2; 3 and 4 measured slower).  Plans made while n > 1 capture one graph per group and replay them
+ * side by side on those streams (16 x 512^2: 9% faster).  Results do not depend on n; concurrent
  * split calls from several host threads share the batch streams.  Process-wide, 1..8, default 2;
  * returns the previous value.  No reference counterpart. */
 int fkadj_set_batch_groups(int n);

@@ -210,7 +210,8 @@ def test_graph_plan_replays_equal_eager():
 def test_split_batches_equal_alone():
     """A batch call of 2M cells or more runs as up to fkadj_set_batch_groups groups, each from its own
     host thread on its own stream: 1, 2 and 3 groups give each instance's own result bit for bit (one
-    member keeps all of its segments, one some); a plan made meanwhile runs one group and agrees."""
+    member keeps all of its segments, one some); a plan made meanwhile (three groups, one captured
+    graph each, replayed side by side) agrees."""
     _print_case("split_batch", sizes="(1024,768),(512,1024),(1024,1024)")
     m = fkadj.reference_model(h=0.025)
     nt, dt, K, ck = 20, 0.02, 4, 10

@@ -5,8 +5,11 @@
 // Reference host routine: fk_grad_Et_comp, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938 (G).
 // Differences by design: the trajectory never leaves HBM (the reference copies every state to
 // host memory and back, G:1721-1723, G:1871-1872), it is kept as checkpoints every K steps and
-// recomputed one segment at a time, the per-step z.gh partials stay on the device until one
-// fixed-order reduction at the end, and errors are returned instead of exit().
+// recomputed one segment at a time (or, within a caller's keep_bytes, kept whole for the last
+// segments), the per-step z.gh partials stay on the device until one fixed-order reduction at the
+// end, and errors are returned instead of exit().  Many instances share each launch (groups); a
+// large batch call runs as two groups from two host threads on two streams, and a continuous-
+// batching server admits requests by their buffers' bytes.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>

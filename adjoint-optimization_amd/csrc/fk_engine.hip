@@ -28,6 +28,7 @@
 
 #include "../../include/fkadj.h"
 #include "fk_internal.hpp"
+#include "fk_tile_map.hpp"
 
 using namespace fk;
 
@@ -745,24 +746,16 @@ int groups_build(const std::vector<EvalCtx *> &members, hipStream_t s, int max_g
                  std::vector<FusedGroupDev *> &out)
 {
     out.clear();
-    auto cells_of = [](const EvalCtx *C) { return (int64_t)C->I->rows * C->I->pitch; };
+    const int n = (int)members.size();
+    std::vector<int64_t> cells(n);
     int64_t total = 0;
-    for (auto *C : members) total += cells_of(C);
-    const int k = total >= kTwoStreamCells ? std::max(1, std::min(max_groups, (int)members.size())) : 1;
+    for (int q = 0; q < n; q++) total += cells[q] = (int64_t)members[q]->I->rows * members[q]->I->pitch;
+    const int k = total >= kTwoStreamCells ? std::max(1, std::min(max_groups, n)) : 1;
+    std::vector<int> end(k);
+    split_runs(cells.data(), n, k, end.data());
     std::vector<std::vector<EvalCtx *>> parts(k);
-    size_t q = 0;
-    int64_t acc = 0;
-    for (int j = 0; j < k; j++) {
-        const int64_t share = total * (j + 1) / k;
-        const size_t last = members.size() - (size_t)(k - 1 - j);   // one member left for each later part
-        while (q < last) {
-            const int64_t c = cells_of(members[q]);
-            if (!parts[j].empty() && acc + c / 2 > share) break;   // its larger half lies beyond this share
-            parts[j].push_back(members[q++]);
-            acc += c;
-        }
-    }
-    for (; q < members.size(); q++) parts[k - 1].push_back(members[q]);
+    for (int j = 0, q = 0; j < k; j++)
+        for (; q < end[j]; q++) parts[j].push_back(members[q]);
     for (auto &part : parts) {
         FusedGroupDev *g = nullptr;
         const int rc = group_build(part, s, &g);

@@ -122,4 +122,26 @@ __host__ __device__ inline void launch_tile_index(int b, const LaunchMap &map, i
     tile = lin - map.first[lo];
 }
 
+// Cut n members with the given cell counts into k contiguous, non-empty runs of about equal cells
+// (1 <= k <= n): member q goes to the current run unless its larger half lies beyond the run's share
+// of the total, keeping one member for each later run.  end[j] is one past the last member of run j.
+inline void split_runs(const int64_t *cells, int n, int k, int *end)
+{
+    int64_t total = 0;
+    for (int q = 0; q < n; q++) total += cells[q];
+    int q = 0;
+    int64_t acc = 0;
+    for (int j = 0; j < k; j++) {
+        const int64_t share = total * (j + 1) / k;
+        const int last = n - (k - 1 - j);
+        const int first = q;
+        while (q < last) {
+            if (q > first && acc + cells[q] / 2 > share) break;
+            acc += cells[q++];
+        }
+        end[j] = q;
+    }
+    end[k - 1] = n;
+}
+
 }  // namespace fk

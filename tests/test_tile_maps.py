@@ -1,7 +1,8 @@
 """The fused kernels' index maps, checked on the host with the kernels' own functions
 (tests/tile_map_check.cpp over csrc/fk_tile_map.hpp, built by __graft_entry__.build()): thread
 ownership of tile and halo cells for every tile configuration and time-blocking depth, stencil
-bounds and closure, slot predicates, the LDS region load map and the XCD-aware launch map."""
+bounds and closure, slot predicates, the LDS region load map, the XCD-aware launch map and the
+split of a large batch call into contiguous runs of members."""
 import os
 import subprocess
 

@@ -8,7 +8,8 @@
 //   * slot predicates: a slot holding a cell of stage s passes slot_in_stage(sl, s) (the kernels
 //     skip whole slots on it), and a slot with a cell finishing an inner step keeps its increments;
 //   * the region load map covers the LDS region exactly once;
-//   * the XCD-aware launch map is a bijection from workgroups onto (slot, tile) for many maps.
+//   * the XCD-aware launch map is a bijection from workgroups onto (slot, tile) for many maps;
+//   * the batch split cuts members into contiguous, non-empty runs of about equal cells.
 // Prints "tile maps ok" and exits 0 when everything holds; prints the failures otherwise.
 #include <cstdio>
 #include <cstdlib>
@@ -120,6 +121,42 @@ static void check_launch_maps()
     }
 }
 
+// split_runs: contiguous, non-empty, covering runs; with two runs, or four members or more per run,
+// every run ends within one member of its share of the cells
+static void check_split_runs()
+{
+    unsigned seed = 777;
+    auto rnd = [&](int n) { seed = seed * 1103515245u + 12345u; return (int)((seed >> 8) % (unsigned)n); };
+    for (int trial = 0; trial < 2000; trial++) {
+        const int n = 1 + rnd(64), k = 1 + rnd(n < 8 ? n : 8);
+        std::vector<int64_t> cells(n);
+        int64_t total = 0, biggest = 0;
+        for (auto &c : cells) {
+            c = trial % 4 == 0 ? 262144 : 4096 * (1 + rnd(256));
+            total += c;
+            biggest = c > biggest ? c : biggest;
+        }
+        std::vector<int> end(k);
+        split_runs(cells.data(), n, k, end.data());
+        int prev = 0;
+        int64_t acc = 0;
+        for (int j = 0; j < k; j++) {
+            CHECK(end[j] > prev, "split %d: run %d of %d empty (n %d)", trial, j, k, n);
+            for (int q = prev; q < end[j]; q++) acc += cells[q];
+            const int64_t share = total * (j + 1) / k;
+            // with few members per run a run may have to take one member whatever its size
+            if (k <= 2 || n >= 4 * k)
+                CHECK(acc - share <= biggest && share - acc <= biggest, "split %d: run %d ends %lld cells from its share",
+                      trial, j, (long long)(acc - share));
+            prev = end[j];
+        }
+        CHECK(end[k - 1] == n, "split %d: runs cover %d of %d members", trial, end[k - 1], n);
+        if (trial % 4 == 0 && n % k == 0)
+            for (int j = 0; j < k; j++)
+                CHECK(end[j] == n / k * (j + 1), "split %d: equal members, run %d ends at %d", trial, j, end[j]);
+    }
+}
+
 int main()
 {
     check_tile<16, 256, 1>();
@@ -131,6 +168,7 @@ int main()
     check_tile<8, 256, 1>();
     check_tile<8, 512, 2>();
     check_launch_maps();
+    check_split_runs();
     if (fails) {
         std::fprintf(stderr, "%d failures\n", fails);
         return 1;

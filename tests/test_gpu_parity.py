@@ -243,6 +243,39 @@ def test_split_batches_equal_alone():
         inst.close()
 
 
+def test_split_plan_at_batch_scale():
+    """Eight 512^2 instances (2.1M cells, so batch calls and plans run two groups): the plan's two
+    captured graphs, replayed side by side three times with eager work between, the split batch call
+    and each instance alone agree bit for bit."""
+    _print_case("split_plan_512x8", nt=200, K=50)
+    from fkadj import workloads as W
+    insts, Es = [], []
+    for k in range(8):
+        mask = W.holes_mask(512, 80, 200 + k)
+        inst = W.make_instance(mask, 4.0, 20, precision=fkadj.F32, ckpt=50)
+        inst.set_state(W.rest_state(int(mask.sum())))
+        s1 = np.zeros(mask.shape, bool)
+        s1[:, :200 + 8 * k] = True
+        inst.stimulate(s1, 1.0)
+        insts.append(inst)
+        Es.append(np.random.default_rng(k).uniform(0, 10, 20))
+    alone = [inst.grad(E) for inst, E in zip(insts, Es)]
+    prev = fkadj.set_batch_groups(2)
+    try:
+        batched = fkadj.batch_grad(insts, Es)
+        with fkadj.Plan(insts) as plan:
+            for rep in range(3):
+                got = plan.grad(Es)
+                for a, b, g in zip(alone, batched, got):
+                    assert np.array_equal(a[0], g[0]) and a[1] == g[1], rep
+                    assert np.array_equal(a[0], b[0]) and a[1] == b[1], rep
+                insts[3].grad(Es[3])                           # eager work between replays
+    finally:
+        fkadj.set_batch_groups(prev)
+    for inst in insts:
+        inst.close()
+
+
 def test_graph_plan_with_kept_segments():
     """A plan over instances keeping all, some and none of their segments replays the eager batched
     results, which equal each instance alone without kept segments, bit for bit."""

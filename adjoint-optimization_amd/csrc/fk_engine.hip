@@ -101,6 +101,17 @@ double spline_eval(const double *t, const double *y, const std::vector<double> &
            (y[hi] / h - ypp[hi] * h / 6.0) * a;
 }
 
+// Active cells of an active_index grid (entries > 0), or -1 for a grid too small to hold a problem.
+// The drop-in entry points compare it with the caller's n_num before anything reads the n_num-sized
+// arrays (gh_num, S), whose lengths the caller sized by its own n_num (G:1759-1762).
+int64_t count_active(const int32_t *active_index, int nrows, int ncols)
+{
+    if (nrows < 1 || ncols < 1) return -1;
+    int64_t n = 0;
+    for (int64_t k = 0; k < (int64_t)nrows * ncols; k++) n += active_index[k] > 0;
+    return n;
+}
+
 bool all_finite(const double *a, size_t n)
 {
     for (size_t i = 0; i < n; i++)
@@ -1274,6 +1285,8 @@ int fkadj_forward_ref(double dt, const double *Et, int nt, const double *y_ini, 
     FKADJ_ENTRY
     if (!Et || !y_ini || !active_index || !gh_num || !yf || nstep < 1 || ns < 0 || nt < 1) return FKADJ_ERR_ARG;
     if (ns > nt / nstep || (ns > 0 && !ys)) return FKADJ_ERR_ARG;
+    if (nrows < 4 || ncols < 4 || n_num < 1) return FKADJ_ERR_ARG;
+    if (count_active(active_index, nrows, ncols) != n_num) return FKADJ_ERR_GEOMETRY;   // before gh_num is read
     fkadj_problem p{};
     p.nrows = nrows;
     p.ncols = ncols;
@@ -1380,6 +1393,10 @@ int fkadj_grad_ref(double dt, const double *Et, int nt, const double *y_ini, con
     FKADJ_ENTRY
     if (!Et || !y_ini || !active_index || !gh_num || !gamma || !S_colptr || !S_rowind || !S_val || !gradE)
         return FKADJ_ERR_ARG;
+    if (nrows < 4 || ncols < 4 || n_num < 1) return FKADJ_ERR_ARG;
+    // S is n_num x n_num and gh_num has n_num values: a caller whose n_num differs from the grid's
+    // active cells gets an error before either is read (G:1759-1762)
+    if (count_active(active_index, nrows, ncols) != n_num) return FKADJ_ERR_GEOMETRY;
     fkadj_problem p{};
     p.nrows = nrows;
     p.ncols = ncols;

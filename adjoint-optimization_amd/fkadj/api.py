@@ -189,10 +189,22 @@ class Instance:
               "fkadj_stimulate")
 
 
+def _fields(instances, Es):
+    """One contiguous float64 field per instance, each of the instance's n_controls values (the C side
+    reads exactly that many from every pointer)."""
+    if len(Es) != len(instances):
+        raise ValueError(f"need {len(instances)} fields, got {len(Es)}")
+    out = [L._f64(np.ravel(E)) for E in Es]
+    for k, (inst, E) in enumerate(zip(instances, out)):
+        if E.size != inst.n_controls:
+            raise ValueError(f"field {k} has {E.size} values, its instance needs {inst.n_controls}")
+    return out
+
+
 def batch_grad(instances, Es):
     """Gradients of several FP32 instances in shared launches; returns a list of (grad, Lcost, L0, LE)."""
     n = len(instances)
-    Es = [L._f64(np.ravel(E)) for E in Es]
+    Es = _fields(instances, Es)
     grads = [np.zeros(inst.n_controls) for inst in instances]
     hs = (C.c_void_p * n)(*[inst.h for inst in instances])
     ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
@@ -216,9 +228,7 @@ class Plan:
 
     def grad(self, Es):
         n = len(self.instances)
-        Es = [L._f64(np.ravel(E)) for E in Es]
-        if len(Es) != n:
-            raise ValueError(f"need {n} fields")
+        Es = _fields(self.instances, Es)
         grads = [np.zeros(inst.n_controls) for inst in self.instances]
         ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
         gp = (C.POINTER(C.c_double) * n)(*[L._d(g) for g in grads])
@@ -244,7 +254,7 @@ class Plan:
 def batch_forward(instances, Es, nsteps):
     """Advance the resident states of several FP32 instances by nsteps in shared launches."""
     n = len(instances)
-    Es = [L._f64(np.ravel(E)) for E in Es]
+    Es = _fields(instances, Es)
     hs = (C.c_void_p * n)(*[inst.h for inst in instances])
     ep = (C.POINTER(C.c_double) * n)(*[L._d(E) for E in Es])
     check(L.lib().fkadj_batch_forward(n, hs, ep, int(nsteps)), "fkadj_batch_forward")

@@ -44,20 +44,6 @@ def build(verbose: bool = False) -> str:
     return LIB
 
 
-def build_probe(name: str) -> str:
-    """A measurement probe under measure/ (copy_probe: HBM-counter calibration; valu_probe: vector-ALU peak)."""
-    src = os.path.join(os.path.dirname(PKG), "measure", name + ".hip")
-    out = src[:-4]
-    r = subprocess.run([HIPCC, *FLAGS, "-o", out, src], capture_output=True, text=True)
-    if r.returncode != 0:
-        raise RuntimeError(f"hipcc failed for {src}:\n{r.stderr}")
-    return out
-
-
-def build_copy_probe() -> str:
-    return build_probe("copy_probe")
-
-
 def build_c_client() -> str:
     """tests/c_abi_client.c, a plain C99 caller of the C ABI (gcc; rpath to this directory)."""
     root = os.path.dirname(os.path.dirname(PKG))

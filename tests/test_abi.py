@@ -209,3 +209,61 @@ def test_forward_ref_argument_errors_before_device_work():
     assert call(3, 4) == L.ERR_ARG            # floor(10/3) = 3 snapshots at most
     assert call(3, 1, ys_ptr=None) == L.ERR_ARG
     assert call(3, 1, yf_ptr=None) == L.ERR_ARG
+
+
+def test_drop_in_entries_reject_n_num_mismatch_before_reading_arrays():
+    """fkadj_grad_ref and fkadj_forward_ref size gh_num and S by the caller's n_num.  An n_num off by
+    one from the grid's active cells returns FKADJ_ERR_GEOMETRY before either array is read (the
+    reference checks S's column count against n_num before using it, G:1759-1762): the arrays here
+    hold exactly the caller's n_num values, so a read past them would be a host overrun."""
+    import ctypes as C
+    lib = fkadj.lib()
+    nr = nc = 8
+    mask = np.ones((nr, nc), bool)
+    ai = G.active_index_from_mask(mask).astype(np.int32)
+    n_act = int(mask.sum())
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    ip = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    aip = ai.ctypes.data_as(C.POINTER(C.c_int32))
+    Et = np.zeros(11)
+    gamma = np.array([1.0, 0.0, 0.0])
+    L0, LE, Lc = C.c_double(), C.c_double(), C.c_double()
+    for n_num in (n_act - 1, n_act + 1):
+        y0 = np.zeros(3 * n_num)
+        gh = np.zeros(n_num)
+        colptr = np.arange(n_num + 1, dtype=np.int64)
+        rowind = np.arange(n_num, dtype=np.int64)
+        val = np.ones(n_num)
+        grad = np.zeros(11)
+        rc = lib.fkadj_grad_ref(0.1, dp(Et), 10, dp(y0), aip, nr, nc, n_num, dp(gh), dp(gamma), ip(colptr),
+                                ip(rowind), dp(val), 0.5, C.byref(L0), C.byref(LE), C.byref(Lc), dp(grad), None, 0)
+        assert rc == L.ERR_GEOMETRY, (n_num, rc)
+        ys = np.zeros(3 * nr * nc)
+        yf = np.zeros(3 * nr * nc)
+        rc = lib.fkadj_forward_ref(0.1, dp(Et), 10, dp(y0), aip, nr, nc, n_num, dp(gh), 5, 1, dp(ys), dp(yf),
+                                   None, 0)
+        assert rc == L.ERR_GEOMETRY, (n_num, rc)
+
+
+def test_batch_calls_check_field_lengths():
+    """batch_grad, Plan.grad and batch_forward reject a field whose length differs from its instance's
+    n_controls (the C side reads that many values from each field) before calling the library."""
+    from fkadj import api
+
+    class Stub:
+        def __init__(self, n):
+            self.n_controls = n
+            self.h = None
+
+    insts = [Stub(4), Stub(6)]
+    with pytest.raises(ValueError):
+        api.batch_grad(insts, [np.zeros(4), np.zeros(5)])
+    with pytest.raises(ValueError):
+        api.batch_forward(insts, [np.zeros(3), np.zeros(6)], 1)
+    with pytest.raises(ValueError):
+        api.batch_grad(insts, [np.zeros(4)])
+    plan = api.Plan.__new__(api.Plan)
+    plan.instances = insts
+    plan.h = None
+    with pytest.raises(ValueError):
+        plan.grad([np.zeros(4), np.zeros(7)])

@@ -168,6 +168,7 @@ struct fkadj_instance {
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
     int fwd_cfg = 0, adj_cfg = TILE_16_512;   // fused tile configurations (fwd 0: per group)
     int keep = 0;         // FP32: last segments the default context keeps from the forward sweep
+    int device = 0;       // GPU the instance lives on (current device at create time)
 
     EvalCtx *ctx = nullptr;   // default context of fkadj_grad / fkadj_forward / batches
 
@@ -539,6 +540,23 @@ void ctx_free(EvalCtx *C)
     delete C;
 }
 
+// The FP32 slot record of a context whose buffers are in place.
+void ctx_fill_slot(EvalCtx *C)
+{
+    const fkadj_instance *I = C->I;
+    FusedSlot &S = C->slot_h;
+    S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
+    S.nt = I->nt; S.K = I->K; S.nck = I->nck;
+    S.ntiles = I->nb_part;
+    S.seg_par = C->seg_copies - 1;
+    S.keep0 = (I->nt + I->K - 1) / I->K - C->keep;
+    S.code = I->d_code;
+    S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
+    S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT); S.kfirst = as<float>(C->d_kfirst);
+    S.Ef = C->d_Ef; S.Ea = C->d_Ea; S.partial = C->d_part;
+    S.m = I->m32; S.dt = (float)I->p.dt;
+}
+
 // Allocates a zeroed context on stream s (hole cells read 0 in every state buffer).  An FP32
 // context also gets its own one-member group (launch maps, a second stream) unless it is only
 // driven through a server (self_group false).
@@ -568,17 +586,7 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
     if (ok && I->prec == FKADJ_F32)
         ok = alc(C->d_kseg, st * nbuf * I->K) && alc(C->d_kT, st) && alc(C->d_kfirst, st * 3);
     if (ok && I->prec == FKADJ_F32) {
-        FusedSlot &S = C->slot_h;
-        S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
-        S.nt = I->nt; S.K = I->K; S.nck = I->nck;
-        S.ntiles = I->nb_part;
-        S.seg_par = C->seg_copies - 1;
-        S.keep0 = (I->nt + I->K - 1) / I->K - C->keep;
-        S.code = I->d_code;
-        S.ck = as<float>(C->d_ck); S.seg = as<float>(C->d_seg); S.ring = as<float>(C->d_ring); S.zb = as<float>(C->d_z);
-        S.kseg = as<float>(C->d_kseg); S.kT = as<float>(C->d_kT); S.kfirst = as<float>(C->d_kfirst);
-        S.Ef = C->d_Ef; S.Ea = C->d_Ea; S.partial = C->d_part;
-        S.m = I->m32; S.dt = (float)I->p.dt;
+        ctx_fill_slot(C);
         if (self_group) {
             std::vector<EvalCtx *> self{C};
             ok = group_build(self, s, &C->group) == FKADJ_OK;
@@ -676,13 +684,12 @@ int fwd_cfg_of(const fkadj_instance *I, int64_t launch_cells)
     return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(launch_cells);
 }
 
-int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
+// Host part of a group: members, launch maps, depth and kept-segment choice; the slot records go to
+// `slots` in member order (no device work).
+void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::vector<FusedSlot> &slots)
 {
-    if (members.empty() || members.size() > (size_t)MAX_SLOTS) return FKADJ_ERR_ARG;
-    auto *g = new (std::nothrow) FusedGroupDev();
-    if (!g) return FKADJ_ERR_NOMEM;
     g->members = members;
-    std::vector<FusedSlot> slots;
+    slots.clear();
     for (int c = 0; c < NUM_TILE_CFG; c++) {
         launch_map_clear(g->fwd[c]);
         launch_map_clear(g->adj[c]);
@@ -706,6 +713,15 @@ int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroup
         launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
         g->depth2 = g->depth2 && I->fwd_cfg == 0;
     }
+}
+
+int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
+{
+    if (members.empty() || members.size() > (size_t)MAX_SLOTS) return FKADJ_ERR_ARG;
+    auto *g = new (std::nothrow) FusedGroupDev();
+    if (!g) return FKADJ_ERR_NOMEM;
+    std::vector<FusedSlot> slots;
+    group_maps(g, members, slots);
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) == hipSuccess &&
@@ -904,18 +920,25 @@ void run_group(FusedGroupDev *g, hipStream_t st, Sampler *smp, GroupRun *out)
     if (out->rc == FKADJ_ERR_DEVICE) memcpy(out->err, g_errmsg, sizeof out->err);
 }
 
+// z.gh (nt+1 values, into the host buffer zgh) and L0 of a finished evaluation, then the gradient.
+int fetch_into_and_assemble(EvalCtx *C, hipStream_t s, double *zgh, const double *E, double *grad, double *Lc,
+                            double *L0, double *LE)
+{
+    fkadj_instance *I = C->I;
+    double L0v = 0.0;
+    HIPCHK(hipMemcpyAsync(zgh, C->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipMemcpyAsync(&L0v, C->d_L0, sizeof(double), hipMemcpyDeviceToHost, s));
+    HIPCHK(hipStreamSynchronize(s));
+    if (!all_finite(zgh, (size_t)I->nt + 1) || !std::isfinite(L0v)) return FKADJ_ERR_NONFINITE;
+    assemble(I, E, zgh, L0v, grad, Lc, L0, LE);
+    return FKADJ_OK;
+}
+
 int fetch_and_assemble(EvalCtx *C, hipStream_t s, const double *E, double *grad, double *Lc, double *L0,
                        double *LE)
 {
-    fkadj_instance *I = C->I;
-    std::vector<double> zgh(I->nt + 1);
-    double L0v = 0.0;
-    HIPCHK(hipMemcpyAsync(zgh.data(), C->d_zgh, sizeof(double) * (I->nt + 1), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipMemcpyAsync(&L0v, C->d_L0, sizeof(double), hipMemcpyDeviceToHost, s));
-    HIPCHK(hipStreamSynchronize(s));
-    if (!all_finite(zgh.data(), zgh.size()) || !std::isfinite(L0v)) return FKADJ_ERR_NONFINITE;
-    assemble(I, E, zgh.data(), L0v, grad, Lc, L0, LE);
-    return FKADJ_OK;
+    std::vector<double> zgh(C->I->nt + 1);
+    return fetch_into_and_assemble(C, s, zgh.data(), E, grad, Lc, L0, LE);
 }
 
 int eval_one(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
@@ -962,6 +985,7 @@ const char *fkadj_strerror(int code)
     case FKADJ_ERR_CANCELLED: return "request cancelled";
     case FKADJ_ERR_BUSY: return "result not ready";
     case FKADJ_ERR_NOTFOUND: return "unknown request";
+    case FKADJ_ERR_SCRATCH: return g_errmsg[0] ? g_errmsg : "evaluation larger than the no-allocation array";
     default: return "unknown error";
     }
 }
@@ -1111,7 +1135,7 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
         if (hipMalloc((void **)&(ptr), (bytes)) != hipSuccess) return fail(FKADJ_ERR_NOMEM); \
     } while (0)
 
-    if (hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking) != hipSuccess) {
+    if (hipGetDevice(&I->device) != hipSuccess || hipStreamCreateWithFlags(&I->stream, hipStreamNonBlocking) != hipSuccess) {
         snprintf(g_errmsg, sizeof g_errmsg, "hipStreamCreate failed (no GPU?)");
         delete I;
         return FKADJ_ERR_DEVICE;
@@ -1528,6 +1552,138 @@ int fkadj_batch_grad(int count, fkadj_instance *const *insts, const double *cons
 }
 
 }  // extern "C"
+
+// ============================================================================ no-allocation evaluation
+// fkadj_grad_noalloc: the evaluation of fkadj_grad with every evaluation buffer (field tables,
+// partials, checkpoints, one segment buffer, ring, adjoint pair, stored slopes or FP64 stage
+// planes, and the slot record) carved out of one static device array instead of hipMalloc'd
+// memory, for callers that cannot allocate device memory.  Each GPU holds its own copy of the array
+// at its own address, taken once per device with hipGetSymbolAddress; host code never passes the
+// array's name to the device.  Calls on one GPU are serialised by a per-device mutex and each call
+// waits for its work, so they never overlap there; the array keeps the last call's contents, and
+// every call clears the part it uses before its first kernel.
+static __device__ __align__(256) char scratch[FKADJ_NOALLOC_SCRATCH_BYTES];
+
+namespace {
+
+struct NoAllocDev {
+    std::mutex mu;                     // serialises calls on this device
+    char *ptr = nullptr;               // device address of `scratch` on this device
+    hipEvent_t ev[5] = {};             // the group's ev_rec[2], ev_adj[2], ev_lane
+    EvalCtx ctx;                       // buffers point into the array; host tables reused
+    FusedGroupDev group;               // the context alone, on the instance's stream
+    std::vector<FusedSlot> slots;
+    std::vector<double> zgh;           // host staging of z.gh
+};
+constexpr int kNoAllocDevices = 64;
+NoAllocDev g_noalloc[kNoAllocDevices];
+
+// Set-up on the current device (first call there): the array's device address and the events.
+int noalloc_setup(NoAllocDev &d)
+{
+    if (d.ptr) return FKADJ_OK;
+    char *p = nullptr;
+    HIPCHK(hipGetSymbolAddress((void **)&p, HIP_SYMBOL(scratch)));
+    for (auto &e : d.ev)
+        if (!e) HIPCHK(hipEventCreateWithFlags(&e, hipEventDisableTiming));
+    d.ptr = p;
+    return FKADJ_OK;
+}
+
+// Carves the context of instance I (one segment buffer, nothing kept) out of [base, base + cap):
+// returns the bytes used (256-byte aligned pieces), or -1 if they exceed cap (then nothing is set
+// beyond that point).  The slot record takes the first piece.
+int64_t noalloc_layout(fkadj_instance *I, char *base, int64_t cap, EvalCtx *C, FusedSlot **d_slot)
+{
+    int64_t off = 0;
+    auto take = [&](int64_t bytes) -> char * {
+        char *p = base + off;
+        off += (std::max<int64_t>(bytes, 1) + 255) / 256 * 256;
+        return off <= cap ? p : nullptr;
+    };
+    const int64_t st = (int64_t)I->st_bytes, nt = I->nt;
+    *d_slot = reinterpret_cast<FusedSlot *>(take(sizeof(FusedSlot)));
+    C->d_Ef = as<double>(take(8 * 3 * nt));
+    C->d_Ea = as<double>(take(8 * 3 * nt));
+    C->d_part = as<double>(take(8 * nt * I->nb_part));
+    C->d_partT = as<double>(take(8 * (int64_t)I->nb_plain));
+    C->d_zgh = as<double>(take(8 * (nt + 1)));
+    C->d_L0part = as<double>(take(8 * (int64_t)objective_blocks(I->n_num)));
+    C->d_L0 = as<double>(take(8));
+    C->d_ck = take(st * I->nck);
+    C->d_seg = take(st * (I->K + 1));
+    C->d_ring = take(2 * st);
+    C->d_z = take(2 * st);
+    C->d_work = I->prec == FKADJ_F64 ? take(6 * st) : nullptr;
+    C->d_kseg = I->prec == FKADJ_F32 ? take(st * I->K) : nullptr;
+    C->d_kT = I->prec == FKADJ_F32 ? take(st) : nullptr;
+    C->d_kfirst = I->prec == FKADJ_F32 ? take(3 * st) : nullptr;
+    return off <= cap ? off : -1;
+}
+
+int noalloc_eval(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0, double *LE)
+{
+    if (I->device < 0 || I->device >= kNoAllocDevices) return FKADJ_ERR_ARG;
+    NoAllocDev &d = g_noalloc[I->device];
+    std::lock_guard<std::mutex> lk(d.mu);
+    CHK(noalloc_setup(d));
+    EvalCtx &C = d.ctx;
+    C.I = I;
+    C.seg_copies = 1;
+    C.keep = 0;
+    C.group = nullptr;
+    FusedSlot *d_slot = nullptr;
+    const int64_t used = noalloc_layout(I, d.ptr, (int64_t)FKADJ_NOALLOC_SCRATCH_BYTES, &C, &d_slot);
+    if (used < 0) {
+        snprintf(g_errmsg, sizeof g_errmsg, "evaluation needs %lld bytes, the no-allocation array holds %lld",
+                 (long long)noalloc_layout(I, nullptr, INT64_MAX, &C, &d_slot),
+                 (long long)FKADJ_NOALLOC_SCRATCH_BYTES);
+        return FKADJ_ERR_SCRATCH;
+    }
+    hipStream_t s = I->stream;
+    // the previous call's contents are still there: hole cells of every state must read 0, and the
+    // objective writes the terminal adjoint only at active cells
+    HIPCHK(hipMemsetAsync(d.ptr, 0, (size_t)used, s));
+    CHK(build_tables(I, &C, E, s));
+    Sampler smp(false);
+    int rc;
+    if (I->prec == FKADJ_F64) {
+        rc = run_plain(&C, smp);
+    } else {
+        ctx_fill_slot(&C);
+        FusedGroupDev &g = d.group;
+        group_maps(&g, std::vector<EvalCtx *>{&C}, d.slots);
+        g.d_slots = d_slot;
+        g.side = s;   // one segment buffer: recompute and adjoint stay on one stream
+        g.ev_rec[0] = d.ev[0]; g.ev_rec[1] = d.ev[1];
+        g.ev_adj[0] = d.ev[2]; g.ev_adj[1] = d.ev[3];
+        g.ev_lane = d.ev[4];
+        // pageable source: the copy is complete when the call returns (the caller's buffer is ours)
+        HIPCHK(hipMemcpyAsync(d_slot, d.slots.data(), sizeof(FusedSlot), hipMemcpyHostToDevice, s));
+        rc = run_fused(&g, s, smp);
+    }
+    if (rc) return rc;
+    d.zgh.resize((size_t)I->nt + 1);
+    return fetch_into_and_assemble(&C, s, d.zgh.data(), E, grad, Lc, L0, LE);
+}
+
+}  // namespace
+
+extern "C" int fkadj_grad_noalloc(fkadj_instance *I, const double *E, double *grad, double *Lc, double *L0,
+                                  double *LE)
+{
+    FKADJ_ENTRY
+    if (!I || !E) return FKADJ_ERR_ARG;
+    if (!I->state_set) return FKADJ_ERR_STATE;
+    if (!all_finite(E, I->ncontrols)) return FKADJ_ERR_NONFINITE;
+    int cur = 0;
+    HIPCHK(hipGetDevice(&cur));
+    if (cur != I->device) HIPCHK(hipSetDevice(I->device));
+    const int rc = noalloc_eval(I, E, grad, Lc, L0, LE);
+    if (cur != I->device) (void)hipSetDevice(cur);
+    return rc;
+    FKADJ_EXIT
+}
 
 // ============================================================================ batch plans
 // A captured HIP graph of one whole evaluation of a fixed group.  The graph holds kernel and

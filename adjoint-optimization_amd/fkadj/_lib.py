@@ -16,6 +16,8 @@ LIB_PATH = os.path.join(PKG, "libfkadj.so")
 OK = 0
 ERR_ARG, ERR_GEOMETRY, ERR_NOMEM, ERR_DEVICE, ERR_NONFINITE, ERR_STATE, ERR_CANCELLED, ERR_BUSY, ERR_NOTFOUND = \
     -1, -2, -3, -4, -5, -6, -7, -8, -9
+ERR_SCRATCH = -10
+NOALLOC_SCRATCH_BYTES = 256 << 20   # FKADJ_NOALLOC_SCRATCH_BYTES
 F64, F32 = 0, 1
 PULSE_KNOTS, PULSE_PIECEWISE = 0, 1
 
@@ -87,6 +89,7 @@ def lib():
     L.fkadj_n_active.argtypes = [vp]
     L.fkadj_n_controls.argtypes = [vp]
     L.fkadj_grad.argtypes = [vp, dp, dp, dp, dp, dp]
+    L.fkadj_grad_noalloc.argtypes = [vp, dp, dp, dp, dp, dp]
     L.fkadj_forward.argtypes = [vp, dp, C.c_int, C.c_int, dp, dp, C.c_int]
     L.fkadj_snapshot_count.argtypes = [C.c_int, C.c_int]
     L.fkadj_forward_ref.argtypes = [C.c_double, dp, C.c_int, dp, ip32, C.c_int, C.c_int, C.c_int, dp, C.c_int,

@@ -172,6 +172,18 @@ class Instance:
         check(L.lib().fkadj_grad(self.h, L._d(E), L._d(g), C.byref(Lc), C.byref(L0), C.byref(LE)), "fkadj_grad")
         return g, Lc.value, L0.value, LE.value
 
+    def grad_noalloc(self, E):
+        """``grad`` through fkadj_grad_noalloc: the evaluation buffers live in the library's static
+        device array (no device allocation); same results.  Returns (grad, Lcost, L0, LE)."""
+        E = L._f64(np.ravel(E))
+        if E.size != self.n_controls:
+            raise ValueError(f"E must have {self.n_controls} values")
+        g = np.zeros(self.n_controls)
+        Lc, L0, LE = C.c_double(), C.c_double(), C.c_double()
+        check(L.lib().fkadj_grad_noalloc(self.h, L._d(E), L._d(g), C.byref(Lc), C.byref(L0), C.byref(LE)),
+              "fkadj_grad_noalloc")
+        return g, Lc.value, L0.value, LE.value
+
     def forward(self, E, nsteps=None, snap_every=0, advance=False):
         """Forward solve; returns (snapshots (m, 3*n_active), final packed state)."""
         E = L._f64(np.ravel(E))

@@ -42,6 +42,7 @@ extern "C" {
 #define FKADJ_ERR_CANCELLED -7  /* request cancelled before completion */
 #define FKADJ_ERR_BUSY      -8  /* no result ready yet (serving poll) */
 #define FKADJ_ERR_NOTFOUND  -9  /* unknown request id */
+#define FKADJ_ERR_SCRATCH  -10  /* evaluation needs more than the no-allocation array holds */
 
 const char *fkadj_strerror(int code);
 
@@ -197,6 +198,28 @@ int fkadj_plan_create(int count, fkadj_instance *const *insts, fkadj_plan **out)
 int fkadj_plan_grad(fkadj_plan *plan, const double *const *E, double *const *grad, double *Lcost, double *L0,
                     double *LE);
 int fkadj_plan_destroy(fkadj_plan *plan);
+
+/* ---------------------------------------------------------------- no-allocation evaluation */
+/* Bytes of the static device array behind fkadj_grad_noalloc (one per GPU the library runs on, held
+ * until the process ends). */
+#define FKADJ_NOALLOC_SCRATCH_BYTES ((size_t)256 << 20)
+
+/* fkadj_grad with no device allocation: no counterpart in the reference.  For callers that cannot
+ * call hipMalloc (tight serving loops, harnesses that forbid allocation).  Same arguments, results
+ * and errors as fkadj_grad, and the same gradient bit for bit; the evaluation buffers (field
+ * tables, z.gh partials, checkpoints every ckpt steps, one segment of K+1 states, ring and adjoint
+ * states, FP32 stored slopes or FP64 stage planes) live in a static device array of
+ * FKADJ_NOALLOC_SCRATCH_BYTES per GPU instead of memory the library allocates.  An evaluation that
+ * needs more (about (nt/ckpt + 2 ckpt + 9) states of 3*nrows*pitch values plus 8*nt*(6 + tiles)
+ * bytes of tables; pitch = ncols rounded up to 64) returns FKADJ_ERR_SCRATCH before any device
+ * work.  The instance keeps its resident data (created with lazy_workspace = 1 it holds nothing
+ * else); nothing is kept across calls: every call clears the part of the array it uses.  Calls on
+ * one GPU share its array: the library serialises them (each waits for the one before), so calls
+ * from several host threads do not race but do not overlap either.  Uses the instance's stream;
+ * the first call on a GPU takes the array's address there and creates the few events the sweep
+ * orders its streams with. */
+int fkadj_grad_noalloc(fkadj_instance *inst, const double *E, double *grad, double *Lcost, double *L0,
+                       double *LE);
 
 /* ---------------------------------------------------------------- timing */
 typedef struct fkadj_timing {

@@ -211,6 +211,11 @@ struct FusedGroupDev {
     // a group of a split batch runs on a batch lane stream (from a host thread of its own), forked
     // from and joined to the caller's stream through ev_lane
     hipEvent_t ev_lane = nullptr;
+    // large groups of one pitch run the row-rolling kernels (one workgroup per strip of roll_hs rows)
+    // for their forward-kind and adjoint steps: the same results as the tiled kernels
+    bool roll = false;
+    int roll_pitch = 0, roll_hs = 16;
+    LaunchMap rollmap;
 };
 
 namespace {
@@ -548,6 +553,7 @@ void ctx_fill_slot(EvalCtx *C)
     S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
     S.nt = I->nt; S.K = I->K; S.nck = I->nck;
     S.ntiles = I->nb_part;
+    S.atile_rows = tile_rows_of(I->adj_cfg);
     S.seg_par = C->seg_copies - 1;
     S.keep0 = (I->nt + I->K - 1) / I->K - C->keep;
     S.code = I->d_code;
@@ -684,6 +690,12 @@ int fwd_cfg_of(const fkadj_instance *I, int64_t launch_cells)
     return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(launch_cells);
 }
 
+// Row-rolling kernels: 0 never, 1 for groups of 2M cells or more (default), 2 for every group that
+// can use them; strip rows (a multiple of 8 and of every member's adjoint tile rows).
+std::atomic<int> g_roll_mode{1};
+std::atomic<int> g_roll_hs{16};
+constexpr int64_t kRollCells = (int64_t)2 << 20;
+
 // Host part of a group: members, launch maps, depth and kept-segment choice; the slot records go to
 // `slots` in member order (no device work).
 void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::vector<FusedSlot> &slots)
@@ -712,6 +724,22 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
         launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
         g->depth2 = g->depth2 && I->fwd_cfg == 0;
+    }
+    // row-rolling kernels: one pitch they support, strips a multiple of every adjoint tile height
+    const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
+    bool roll = rmode == 2 || (rmode == 1 && cells >= kRollCells);
+    for (size_t k = 0; k < members.size() && roll; k++) {
+        const fkadj_instance *I = members[k]->I;
+        roll = I->pitch == I0->pitch && roll_pitch_supported(I->pitch) && hs % tile_rows_of(I->adj_cfg) == 0;
+    }
+    g->roll = roll;
+    g->roll_pitch = I0->pitch;
+    g->roll_hs = hs;
+    launch_map_clear(g->rollmap);
+    if (roll) {
+        g->depth2 = false;
+        for (size_t k = 0; k < members.size(); k++)
+            launch_map_add(g->rollmap, (int)k, (members[k]->I->rows + hs - 1) / hs);
     }
 }
 
@@ -802,6 +830,10 @@ Cursors uniform_cursors(int count, int mode, int step, int seg_a)
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
+    if (g->roll && mode != MODE_RHS) {
+        roll_launch(false, g->roll_pitch, g->d_slots, g->rollmap, c, g->roll_hs, s);
+        return;
+    }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
 }
 
@@ -814,6 +846,10 @@ void group_forward2(const FusedGroupDev *g, int mode, int step, int seg_a, hipSt
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
+    if (g->roll) {
+        roll_launch(true, g->roll_pitch, g->d_slots, g->rollmap, c, g->roll_hs, s);
+        return;
+    }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj[k], c, s);
 }
 
@@ -1453,6 +1489,13 @@ int fkadj_set_batch_groups(int n)
 {
     if (n < 1 || n > 8) return FKADJ_ERR_ARG;
     return g_batch_groups.exchange(n);
+}
+
+int fkadj_set_rolling(int mode, int strip_rows)
+{
+    if (mode < 0 || mode > 2 || strip_rows < 8 || strip_rows > 256 || strip_rows % 8 != 0) return FKADJ_ERR_ARG;
+    g_roll_hs.store(strip_rows);
+    return g_roll_mode.exchange(mode);
 }
 
 int fkadj_set_profiling(int on)

@@ -104,6 +104,7 @@ struct FusedSlot {
     int rows, pitch, n;
     int nt, K, nck;             // steps, checkpoint interval, checkpoints
     int ntiles;                 // tiles of this instance (zgh partial row length)
+    int atile_rows;             // rows of the instance's adjoint tiles (64 x atile_rows cells per partial)
     int seg_par;                // 1: seg and kseg hold two buffers, chosen by segment parity; 0: one
     int keep0;                  // first kept segment (segments keep0.. have buffers of their own,
                                 // filled by the forward sweep); the segment count when none is kept
@@ -178,6 +179,12 @@ void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
 void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream);
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream);
+// Row-rolling kernels (fk_kernels_roll.hip) for instances of one pitch (256, 384, 512, 768 or 1024):
+// one workgroup per strip of Hs rows (launch tiles are strips), Hs a multiple of every member's
+// adjoint tile rows; same results as the tiled kernels bit for bit.
+bool roll_pitch_supported(int pitch);
+void roll_launch(bool adjoint, int pitch, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, int Hs,
+                 hipStream_t stream);
 
 // ---- common small kernels (fk_kernels_common.hip)
 // z (terminal adjoint, T planes) and L0 partials from y(T): CSR S over packed indices or s_diag*I.

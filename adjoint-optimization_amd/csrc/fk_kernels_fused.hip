@@ -25,45 +25,15 @@
 
 #include "fk_internal.hpp"
 #include "fk_tile_map.hpp"
+#include "fk_fused_math.hpp"
 
 namespace fk {
 
-// Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0 in
-// LDS, so the mirror Laplacian (a missing neighbour takes the opposite one's value) is
-// sum_w = (1+[b missing]) a + (1+[a missing]) b per axis, and L^T uses its 0/1/2 weights directly.
-struct LapW {
-    float t, b, l, r;   // weights of the top (i-1), bottom (i+1), left (j-1), right (j+1) values
-    float gh;           // 2 Du/h * ghost_lr
-};
-
-__device__ inline LapW lap_weights(const Model32 &m, uint32_t c)
-{
-    LapW w;
-    w.t = (c & C_BOT_MISS) ? 2.0f : 1.0f;
-    w.b = (c & C_TOP_MISS) ? 2.0f : 1.0f;
-    w.l = (c & C_RIGHT_MISS) ? 2.0f : 1.0f;
-    w.r = (c & C_LEFT_MISS) ? 2.0f : 1.0f;
-    w.gh = m.ghc * (float)code_gh(c);
-    return w;
-}
-
-__device__ inline LapW lapT_weights(uint32_t c)
-{
-    LapW w;
-    w.t = (float)((c >> C_WT_SHIFT) & 3u);
-    w.b = (float)((c >> C_WB_SHIFT) & 3u);
-    w.l = (float)((c >> C_WL_SHIFT) & 3u);
-    w.r = (float)((c >> C_WR_SHIFT) & 3u);
-    w.gh = 0.0f;
-    return w;
-}
-
-// sum of weighted neighbours minus 4 x centre (grid units)
+// sum of weighted neighbours minus 4 x centre (grid units) from the LDS tile
 template <int RY, int RX>
 __device__ inline float stencil(const float (*s)[RY][RX], int v, int ly, int lx, const LapW &w, float ctr)
 {
-    return fmaf(w.t, s[v][ly - 1][lx], fmaf(w.b, s[v][ly + 1][lx],
-                fmaf(w.l, s[v][ly][lx - 1], fmaf(w.r, s[v][ly][lx + 1], -4.0f * ctr))));
+    return stencil5(w, s[v][ly - 1][lx], s[v][ly + 1][lx], s[v][ly][lx - 1], s[v][ly][lx + 1], ctr);
 }
 
 // f(y) + E gh at an active cell from the LDS tile
@@ -71,14 +41,16 @@ template <int RY, int RX>
 __device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], const LapW &w, int ly, int lx, float E,
                                float *f, float *yc)
 {
+    float top[3], bot[3], lft[3], rgt[3];
 #pragma unroll
-    for (int v = 0; v < 3; v++) yc[v] = s[v][ly][lx];
-    Gates32 g = gates32(m, yc[0]);
-    float fu, fv, fw;
-    fion32(m, g, yc[0], yc[1], yc[2], fu, fv, fw);
-    f[0] = fmaf(m.lc[0], stencil<RY, RX>(s, 0, ly, lx, w, yc[0]), fmaf(E, w.gh, fu));
-    f[1] = fmaf(m.lc[1], stencil<RY, RX>(s, 1, ly, lx, w, yc[1]), fv);
-    f[2] = fmaf(m.lc[2], stencil<RY, RX>(s, 2, ly, lx, w, yc[2]), fw);
+    for (int v = 0; v < 3; v++) {
+        yc[v] = s[v][ly][lx];
+        top[v] = s[v][ly - 1][lx];
+        bot[v] = s[v][ly + 1][lx];
+        lft[v] = s[v][ly][lx - 1];
+        rgt[v] = s[v][ly][lx + 1];
+    }
+    rhs_vals(m, w, yc, top, bot, lft, rgt, E, f);
 }
 
 // L^T z at an active cell from the LDS tile (missing neighbours have weight 0)
@@ -91,13 +63,6 @@ __device__ inline void lapT_lds(const Model32 &m, const float (*s)[RY][RX], cons
         zc[v] = s[v][ly][lx];
         lt[v] = m.lc[v] * stencil<RY, RX>(s, v, ly, lx, w, zc[v]);
     }
-}
-
-__device__ inline void jtz(const Jac32 &J, const float *z, float *o)
-{
-    o[0] = J.a00 * z[0] + J.a01 * z[1] + J.a02 * z[2];
-    o[1] = J.a10 * z[0] + J.a11 * z[1];
-    o[2] = J.a20 * z[0] + J.a22 * z[2];
 }
 
 // Launch tile of this workgroup (XCD-aware order: launch_tile_index).
@@ -328,21 +293,6 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_fwd(const FusedSlot *__restri
 }
 
 // ------------------------------------------------------------------------------ adjoint
-template <typename T>
-__device__ inline T h3(T x0, T x1, T m0, T m1)
-{
-    constexpr double s = 1.0 / 3.0;
-    return T((s - 1.0) * (s - 1.0) * (2.0 * s + 1.0)) * x0 + T(s * (s - 1.0) * (s - 1.0)) * m0 +
-           T(s * s * (3.0 - 2.0 * s)) * x1 + T(s * s * (s - 1.0)) * m1;
-}
-template <typename T>
-__device__ inline T h4p(T x0, T x1, T m0, T m1)
-{
-    constexpr double s = 0.5;
-    return T((1.0 + 2.0 * s + 9.0 * s * s) * (s - 1.0) * (s - 1.0)) * x0 + T(s * (s - 1.0) * (s - 1.0)) * m0 +
-           T(-s * s * (6.0 - 16.0 * s + 9.0 * s * s)) * x1 + T((s - 1.0) * (9.0 * s - 5.0) * s * s / 4.0) * m1;
-}
-
 template <int TY, int NT, int WPE>
 __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
 {
@@ -515,8 +465,9 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     }
     __syncthreads();
 
-    // phase 4: adjoint stage 4 on the tile with J(y_{i-1}); new z and its z.gh partial
-    double part = 0.0;
+    // phase 4: adjoint stage 4 on the tile with J(y_{i-1}); new z and its z.gh partial: each wavefront
+    // holds one tile row per slot; the row sums (wave_sum64) are added in row order from 0.0
+    __shared__ double sP[TY];
 #pragma unroll
     for (int sl = 0; sl < TS; sl++) {
         float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
@@ -529,15 +480,20 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
             for (int v = 0; v < 3; v++) zn[v] = z[sl][v] + (dz[sl][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
         }
         int gy = t.y0 + ly[sl] - G::H;
-        if (gy < S.rows) {
+        const bool in = gy < S.rows;
+        if (in) {
 #pragma unroll
             for (int v = 0; v < 3; v++) zout[v * S.n + kk[sl]] = zn[v];
-            int gh = code_gh(cc[sl]);
-            if (act[sl] && gh != 0) part += (double)zn[0] * ((double)m.ghc * (double)gh);
         }
+        const double rs = wave_sum64(zgh_term(m, act[sl] && in, cc[sl], zn[0]));
+        if ((tid & 63) == 0) sP[ly[sl] - G::H] = rs;
     }
-    double s = block_sum<NT>(part, tid);
-    if (tid == 0) gmem(S.partial)[(size_t)row * S.ntiles + t.tile] = s;
+    __syncthreads();
+    if (tid == 0) {
+        double s = 0.0;
+        for (int j = 0; j < TY; j++) s += sP[j];
+        gmem(S.partial)[(size_t)row * S.ntiles + t.tile] = s;
+    }
 }
 
 // ------------------------------------------------------------------------------ launchers

@@ -109,6 +109,7 @@ def lib():
     L.fkadj_server_pending.argtypes = [vp]
     L.fkadj_set_profiling.argtypes = [C.c_int]
     L.fkadj_set_batch_groups.argtypes = [C.c_int]
+    L.fkadj_set_rolling.argtypes = [C.c_int, C.c_int]
     L.fkadj_last_timing.argtypes = [C.POINTER(Timing)]
     _lib = L
     return L
@@ -158,6 +159,14 @@ def set_batch_groups(n: int) -> int:
     previous setting."""
     rc = lib().fkadj_set_batch_groups(int(n))
     check(min(rc, 0), "fkadj_set_batch_groups")
+    return rc
+
+
+def set_rolling(mode: int, strip_rows: int = 16) -> int:
+    """Row-rolling kernels for large one-pitch FP32 groups: 0 never, 1 from 2M cells (default), 2
+    whenever possible; strip_rows a multiple of 8.  Returns the previous mode."""
+    rc = lib().fkadj_set_rolling(int(mode), int(strip_rows))
+    check(min(rc, 0), "fkadj_set_rolling")
     return rc
 
 

@@ -267,3 +267,18 @@ def test_batch_calls_check_field_lengths():
     plan.h = None
     with pytest.raises(ValueError):
         plan.grad([np.zeros(4), np.zeros(7)])
+
+
+def test_rolling_setting():
+    """The row-rolling setting takes modes 0..2 and strip heights that are multiples of 8 (no device
+    work) and returns the previous mode."""
+    prev = fkadj.set_rolling(2, 32)
+    try:
+        assert prev in (0, 1, 2)
+        assert fkadj.set_rolling(0, 8) == 2
+        for mode, hs in ((3, 16), (-1, 16), (1, 12), (1, 0), (1, 264)):
+            with pytest.raises(fkadj.FkadjError) as e:
+                fkadj.set_rolling(mode, hs)
+            assert e.value.code == L.ERR_ARG
+    finally:
+        fkadj.set_rolling(prev, 16)

@@ -27,16 +27,35 @@
 
 namespace fk {
 
-// Two-row LDS ring of three planes, each row P values between two zero pads (the values of the
-// missing neighbours at the grid's left and right edges).
+// Two-row LDS rings of three planes, interleaved by column: the values of column c (every ring,
+// both row parities, the three planes) are CS consecutive floats (CS odd: lanes reading the same
+// slot of consecutive columns hit distinct banks), so one base register per lane addresses every
+// ring value and its left and right neighbours with immediate offsets.  Columns -1 and P are zero
+// pads (the values of the missing neighbours at the grid's left and right edges).
 #define FK_LDS __attribute__((address_space(3)))
-template <int P>
-struct Ring2 {
-    static constexpr int PP = P + 2;
-    static constexpr int FLOATS = 2 * 3 * PP;
-    FK_LDS float *b;
-    __device__ FK_LDS float &at(int par, int v, int c) const { return b[(par * 3 + v) * PP + 1 + c]; }
+template <int P, int NRINGS>
+struct Rings {
+    static constexpr int CS = 6 * NRINGS + 1;
+    static constexpr int FLOATS = (P + 2) * CS;
+    FK_LDS float *b;   // this lane's column: lds + (c + 1) * CS
+    __device__ FK_LDS float &at(int ring, int par, int v, int dc) const { return b[dc * CS + ring * 6 + par * 3 + v]; }
 };
+
+// Model constants that meet a second scalar operand in a fused multiply-add, held in vector
+// registers for the whole kernel (a VOP3 instruction reads at most one scalar register; the
+// compiler would otherwise copy them again for every cell evaluation).  The values are unchanged.
+__device__ inline float pin_v(float x)
+{
+    asm volatile("" : "+v"(x));
+    return x;
+}
+__device__ inline void pin_model(Model32 &m)
+{
+    m.bq = pin_v(m.bq);
+    m.bs = pin_v(m.bs);
+    m.dtv = pin_v(m.dtv);
+    m.i_tau_r = pin_v(m.i_tau_r);
+}
 
 // Lap weights of a row cell packed one per byte (top, bottom, left, right; 1 or 2), 0 for an
 // inactive cell: one word per row instead of four registers, unpacked by byte conversions.
@@ -64,14 +83,33 @@ __device__ inline LapW unpack_w(uint32_t w, float gh)
     return r;
 }
 
-// Row loads: wave-uniform row base (scalar registers) plus the lane's column.
-__device__ inline float ld_row(const float *plane, int row, int P, int c)
+// Row accesses through buffer resources: a per-lane 32-bit byte offset (the column, a VGPR held
+// for the whole kernel) and a wave-uniform byte offset of the row and plane (a scalar register),
+// so an access costs no vector address arithmetic.  Every access is in bounds (rows are checked
+// before, columns are the pitch), so the resource's range is the whole 31-bit offset space.
+__device__ inline __amdgpu_buffer_rsrc_t buf_rsrc(const void *p)
 {
-    return ld_at(gmem(uniform_ptr(plane + (size_t)row * P)), (uint32_t)c);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, 0x7fffffff, 0x00020000);
 }
-__device__ inline void st_row(float *plane, int row, int P, int c, float v)
+// three planes (n cells apart) of row `row` of a state-shaped buffer; c4 = 4 x column
+template <int P>
+__device__ inline void ld3(__amdgpu_buffer_rsrc_t r, int row, int n, uint32_t c4, float *out)
 {
-    st_at(gmem(uniform_ptr(plane + (size_t)row * P)), (uint32_t)c, v);
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+        out[v] = __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, c4, (v * n + row * P) * 4, 0));
+}
+template <int P>
+__device__ inline void st3(__amdgpu_buffer_rsrc_t r, int row, int n, uint32_t c4, const float *val)
+{
+#pragma unroll
+    for (int v = 0; v < 3; v++)
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, val[v]), r, c4, (v * n + row * P) * 4, 0);
+}
+template <int P>
+__device__ inline uint32_t ld_code(__amdgpu_buffer_rsrc_t r, int row, uint32_t c2)
+{
+    return (uint32_t)__builtin_amdgcn_raw_buffer_load_b16(r, c2, row * P * 2, 0);
 }
 
 // ------------------------------------------------------------------------------ forward
@@ -85,10 +123,10 @@ struct RollFwd {
     Model32 m;
     float dt, a12, a3, b1, b23, b4, E1, E23, E4;
     int rows, n, c, y0, y1, tb, mode;
-    const float *yin;
-    float *yout, *kout, *kout2;
-    const uint16_t *code;
-    Ring2<P> Ly, L2, L3, L4;
+    __amdgpu_buffer_rsrc_t yin, yout, kout, kout2, code;
+    bool has_k, has_k2;      // slopes stored (MODE_REC, MODE_KEEP), and a second copy (first step of a segment)
+    Rings<P, 4> L;        // rings 0..3: state y, stage inputs Y2, Y3, Y4
+    uint32_t c4, c2;      // byte offsets of the lane's column in a float and in a code plane
     float y[6][3];        // state, rows t-3 .. t+1
     uint32_t w[6];        // packed lap weights, rows t-3 .. t+1
     float gh[6];
@@ -100,9 +138,8 @@ struct RollFwd {
     __device__ void load(int row)
     {
         if ((unsigned)row < (unsigned)rows) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) yn[v] = ld_row(yin + (size_t)v * n, row, P, c);
-            cn = ld_at(gmem(uniform_ptr(code + (size_t)row * P)), (uint32_t)c);
+            ld3<P>(yin, row, n, c4, yn);
+            cn = ld_code<P>(code, row, c2);
         } else {
             yn[0] = yn[1] = yn[2] = 0.0f;
             cn = 0u;
@@ -124,7 +161,7 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
 #pragma unroll
     for (int v = 0; v < 3; v++) {
         F.y[qa][v] = F.yn[v];
-        F.Ly.at(p1, v, c) = F.yn[v];
+        F.L.at(0, p1, v, 0) = F.yn[v];
     }
     F.w[qa] = pack_lap(F.cn);
     F.gh[qa] = F.m.ghc * (float)code_gh(F.cn);
@@ -136,15 +173,12 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
         if ((unsigned)t < (unsigned)F.rows && F.w[q0] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
-            for (int v = 0; v < 3; v++) { lf[v] = F.Ly.at(p0, v, c - 1); rt[v] = F.Ly.at(p0, v, c + 1); }
+            for (int v = 0; v < 3; v++) { lf[v] = F.L.at(0, p0, v, -1); rt[v] = F.L.at(0, p0, v, 1); }
             rhs_vals(F.m, unpack_w(F.w[q0], F.gh[q0]), F.y[q0], F.y[qm1], F.y[qa], lf, rt, F.E1, f);
         }
-        if (F.kout && t >= F.y0 && t < F.y1) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) st_row(F.kout + (size_t)v * F.n, t, P, c, f[v]);
-            if (F.kout2)
-#pragma unroll
-                for (int v = 0; v < 3; v++) st_row(F.kout2 + (size_t)v * F.n, t, P, c, f[v]);
+        if (F.has_k && t >= F.y0 && t < F.y1) {
+            st3<P>(F.kout, t, F.n, F.c4, f);
+            if (F.has_k2) st3<P>(F.kout2, t, F.n, F.c4, f);
         }
 #pragma unroll
         for (int v = 0; v < 3; v++) {
@@ -159,7 +193,7 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
         if ((unsigned)row < (unsigned)F.rows && F.w[qm1] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
-            for (int v = 0; v < 3; v++) { lf[v] = F.L2.at(p1, v, c - 1); rt[v] = F.L2.at(p1, v, c + 1); }
+            for (int v = 0; v < 3; v++) { lf[v] = F.L.at(1, p1, v, -1); rt[v] = F.L.at(1, p1, v, 1); }
             rhs_vals(F.m, unpack_w(F.w[qm1], F.gh[qm1]), F.Y2[rm1], F.Y2[rm2], F.Y2[r0], lf, rt, F.E23, f);
         }
 #pragma unroll
@@ -175,7 +209,7 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
         if ((unsigned)row < (unsigned)F.rows && F.w[qm2] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
-            for (int v = 0; v < 3; v++) { lf[v] = F.L3.at(p0, v, c - 1); rt[v] = F.L3.at(p0, v, c + 1); }
+            for (int v = 0; v < 3; v++) { lf[v] = F.L.at(2, p0, v, -1); rt[v] = F.L.at(2, p0, v, 1); }
             rhs_vals(F.m, unpack_w(F.w[qm2], F.gh[qm2]), F.Y3[rm2], F.Y3[rm3], F.Y3[rm1], lf, rt, F.E23, f);
         }
 #pragma unroll
@@ -191,21 +225,22 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
         if (F.w[qm3] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
-            for (int v = 0; v < 3; v++) { lf[v] = F.L4.at(p1, v, c - 1); rt[v] = F.L4.at(p1, v, c + 1); }
+            for (int v = 0; v < 3; v++) { lf[v] = F.L.at(3, p1, v, -1); rt[v] = F.L.at(3, p1, v, 1); }
             rhs_vals(F.m, unpack_w(F.w[qm3], F.gh[qm3]), F.Y4[rm3], F.Y4[rm4], F.Y4[rm2], lf, rt, F.E4, f);
         }
         const bool act = F.w[qm3] != 0u;
+        float yo[3];
 #pragma unroll
-        for (int v = 0; v < 3; v++)
-            st_row(F.yout + (size_t)v * F.n, row, P, c, act ? F.y[qm3][v] + (F.acc[rm3][v] + F.b4 * f[v]) : 0.0f);
+        for (int v = 0; v < 3; v++) yo[v] = act ? F.y[qm3][v] + (F.acc[rm3][v] + F.b4 * f[v]) : 0.0f;
+        st3<P>(F.yout, row, F.n, F.c4, yo);
     }
     // this iteration's stage values for the neighbours' stencils in the next one: each ring slot
     // held the row two above, read above in this lane's own column only
 #pragma unroll
     for (int v = 0; v < 3; v++) {
-        F.L2.at(p0, v, c) = F.Y2[r0][v];
-        if (t >= F.tb + 2) F.L3.at(p1, v, c) = F.Y3[rm1][v];
-        if (t >= F.tb + 4) F.L4.at(p0, v, c) = F.Y4[rm2][v];
+        F.L.at(1, p0, v, 0) = F.Y2[r0][v];
+        if (t >= F.tb + 2) F.L.at(2, p1, v, 0) = F.Y3[rm1][v];
+        if (t >= F.tb + 4) F.L.at(3, p0, v, 0) = F.Y4[rm2][v];
     }
     __syncthreads();
 }
@@ -228,22 +263,32 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_fwd(const FusedSlot *__rest
     F.mode = sc.mode;
     const int step = sc.step, seg_a = sc.seg_a;
     const size_t st = (size_t)3 * S.n;
-    F.kout = nullptr;
-    F.kout2 = nullptr;
+    const float *yin;
+    float *yout, *kout = S.kT, *kout2 = S.kT;   // kT: a valid address for the unused resources
+    F.has_k = F.has_k2 = false;
     if (F.mode == MODE_FWD) {
-        F.yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
+        yin = (step % S.K == 0) ? S.ck + (size_t)(step / S.K) * st : S.ring + (size_t)(step & 1) * st;
         const int nx = step + 1;
-        F.yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
-                                                : S.ring + (size_t)(nx & 1) * st;
+        yout = (nx % S.K == 0 || nx == S.nt) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
+                                              : S.ring + (size_t)(nx & 1) * st;
     } else {
         const int j = step - seg_a, segi = seg_a / S.K, buf = seg_buf(S, segi), nx = step + 1;
         float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
-        F.yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
-        F.yout = (F.mode == MODE_KEEP && (nx % S.K == 0 || nx == S.nt)) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
-                                                                       : segb + (size_t)(j + 1) * st;
-        F.kout = S.kseg + ((size_t)buf * S.K + j) * st;
-        if (F.mode == MODE_REC && j == 0 && segi < S.keep0) F.kout2 = S.kfirst + (size_t)(segi % 3) * st;
+        yin = (j == 0) ? S.ck + (size_t)segi * st : segb + (size_t)j * st;
+        yout = (F.mode == MODE_KEEP && (nx % S.K == 0 || nx == S.nt)) ? S.ck + (size_t)((nx + S.K - 1) / S.K) * st
+                                                                     : segb + (size_t)(j + 1) * st;
+        kout = S.kseg + ((size_t)buf * S.K + j) * st;
+        F.has_k = true;
+        if (F.mode == MODE_REC && j == 0 && segi < S.keep0) {
+            kout2 = S.kfirst + (size_t)(segi % 3) * st;
+            F.has_k2 = true;
+        }
     }
+    F.yin = buf_rsrc(yin);
+    F.yout = buf_rsrc(yout);
+    F.kout = buf_rsrc(kout);
+    F.kout2 = buf_rsrc(kout2);
+    F.code = buf_rsrc(S.code);
     F.E1 = (float)S.Ef[3 * step];
     F.E23 = (float)S.Ef[3 * step + 1];
     F.E4 = (float)S.Ef[3 * step + 2];
@@ -257,17 +302,17 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_fwd(const FusedSlot *__rest
     F.b4 = F.dt * (1.0f / 6.0f);
     F.rows = S.rows;
     F.n = S.n;
-    F.code = S.code;
     F.c = threadIdx.x;
+    F.c4 = (uint32_t)F.c * 4u;
+    F.c2 = (uint32_t)F.c * 2u;
     F.y0 = strip * Hs;
     F.y1 = min(F.y0 + Hs, S.rows);
     F.tb = F.y0 - 3;
-    F.Ly.b = lds;
-    F.L2.b = lds + Ring2<P>::FLOATS;
-    F.L3.b = lds + 2 * Ring2<P>::FLOATS;
-    F.L4.b = lds + 3 * Ring2<P>::FLOATS;
-    // zero pads: 4 rings x 2 rows x 3 planes, two pads each
-    for (int q = F.c; q < 48; q += P) lds[(q >> 1) * Ring2<P>::PP + ((q & 1) ? Ring2<P>::PP - 1 : 0)] = 0.0f;
+    F.L.b = lds + (F.c + 1) * Rings<P, 4>::CS;
+    pin_model(F.m);
+    // zero pads: columns -1 and P
+    constexpr int CSF = Rings<P, 4>::CS;
+    for (int q = F.c; q < 2 * CSF; q += P) lds[q < CSF ? q : (P + 1) * CSF + q - CSF] = 0.0f;
     // rows tb-1 and tb before the first iteration (row tb also for its neighbours), row tb+1 in flight
     F.load(F.tb - 1);
     constexpr int q_m1 = 5, q_0 = 0;   // ring slots of rows tb-1 and tb
@@ -279,7 +324,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_fwd(const FusedSlot *__rest
 #pragma unroll
     for (int v = 0; v < 3; v++) {
         F.y[q_0][v] = F.yn[v];
-        F.Ly.at(0, v, F.c) = F.yn[v];
+        F.L.at(0, 0, v, 0) = F.yn[v];
     }
     F.w[q_0] = pack_lap(F.cn);
     F.gh[q_0] = F.m.ghc * (float)code_gh(F.cn);
@@ -327,11 +372,10 @@ struct RollAdj {
     Model32 m;
     float dt, E3, dEB;
     int rows, n, c, y0, y1, tb, tyA, ntiles, row_i;
-    const float *zin, *yA, *yB, *kA, *kB;
-    float *zout;
+    __amdgpu_buffer_rsrc_t zin, yA, yB, kA, kB, zout, code;
     double *partial;
-    const uint16_t *code;
-    Ring2<P> Xz, X13, X1, X2, X3;
+    Rings<P, 5> X;        // rings 0..4: z, y(t_{i-1} + dt/3), Z1, Z2, Z3
+    uint32_t c4, c2;      // byte offsets of the lane's column in a float and in a code plane
     float z[6][3];        // adjoint state, rows t-3 .. t+1
     float dz[6][3];       // RK sums, rows t-3 .. t
     uint32_t cd[6];       // cell codes, rows t-3 .. t
@@ -346,20 +390,16 @@ struct RollAdj {
     __device__ void load(int row)   // the arrivals of iteration `row`
     {
         if ((unsigned)(row + 1) < (unsigned)rows) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) zn[v] = ld_row(zin + (size_t)v * n, row + 1, P, c);
+            ld3<P>(zin, row + 1, n, c4, zn);
         } else {
             zn[0] = zn[1] = zn[2] = 0.0f;
         }
         if ((unsigned)row < (unsigned)rows) {
-#pragma unroll
-            for (int v = 0; v < 3; v++) {
-                an[v] = ld_row(yA + (size_t)v * n, row, P, c);
-                bn[v] = ld_row(yB + (size_t)v * n, row, P, c);
-                kan[v] = ld_row(kA + (size_t)v * n, row, P, c);
-                kbn[v] = ld_row(kB + (size_t)v * n, row, P, c);
-            }
-            cn = ld_at(gmem(uniform_ptr(code + (size_t)row * P)), (uint32_t)c);
+            ld3<P>(yA, row, n, c4, an);
+            ld3<P>(yB, row, n, c4, bn);
+            ld3<P>(kA, row, n, c4, kan);
+            ld3<P>(kB, row, n, c4, kbn);
+            cn = ld_code<P>(code, row, c2);
         } else {
 #pragma unroll
             for (int v = 0; v < 3; v++) an[v] = bn[v] = kan[v] = kbn[v] = 0.0f;
@@ -378,16 +418,14 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
     constexpr int h0 = PH & 1, h1 = (PH + 1) & 1;   // mod-2 slots of rows t (t-2) and t-1; also LDS parities
     constexpr int p0 = h0, p1 = h1;
     // y_{i-1} of row t-3 for stage 4 (in the grid whenever stage 4 runs)
-    if (t >= A.tb + 6)
-#pragma unroll
-        for (int v = 0; v < 3; v++) A.ar[v] = ld_row(A.yA + (size_t)v * A.n, t - 3, P, c);
+    if (t >= A.tb + 6) ld3<P>(A.yA, t - 3, A.n, A.c4, A.ar);
     // arrivals: z of row t+1 (its ring slot held row t-1, whose neighbours' values were read in the
     // previous iteration), the code of row t
     const float zup[3] = {A.z[qm1][0], A.z[qm1][1], A.z[qm1][2]};
 #pragma unroll
     for (int v = 0; v < 3; v++) {
         A.z[qa][v] = A.zn[v];
-        A.Xz.at(p1, v, c) = A.zn[v];
+        A.X.at(0, p1, v, 0) = A.zn[v];
     }
     A.cd[q0] = A.cn;
     const uint32_t cc0 = A.cn;
@@ -414,7 +452,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         float lt[3];
 #pragma unroll
         for (int v = 0; v < 3; v++)
-            lt[v] = m.lc[v] * stencil5(tw, zup[v], A.z[qa][v], A.Xz.at(p0, v, c - 1), A.Xz.at(p0, v, c + 1), A.z[q0][v]);
+            lt[v] = m.lc[v] * stencil5(tw, zup[v], A.z[qa][v], A.X.at(0, p0, v, -1), A.X.at(0, p0, v, 1), A.z[q0][v]);
         A.dz[q0][0] = A.dz[q0][1] = A.dz[q0][2] = 0.0f;
         if (act0) {
             Gates32 g = gates32(m, A.bn[0]);
@@ -438,17 +476,17 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         float lt[3], zc[3];
 #pragma unroll
         for (int v = 0; v < 3; v++) {
-            zc[v] = A.X1.at(p1, v, c);
-            lt[v] = m.lc[v] * stencil5(tw, A.X1.at(p0, v, c), Z1[v], A.X1.at(p1, v, c - 1), A.X1.at(p1, v, c + 1), zc[v]);
+            zc[v] = A.X.at(2, p1, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, A.X.at(2, p0, v, 0), Z1[v], A.X.at(2, p1, v, -1), A.X.at(2, p1, v, 1), zc[v]);
         }
         if (cc & C_ACTIVE) {
             float yc[3], top[3], lf[3], rt[3], f3[3], yh[3], o[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) {
-                yc[v] = A.X13.at(p1, v, c);
-                top[v] = A.X13.at(p0, v, c);
-                lf[v] = A.X13.at(p1, v, c - 1);
-                rt[v] = A.X13.at(p1, v, c + 1);
+                yc[v] = A.X.at(1, p1, v, 0);
+                top[v] = A.X.at(1, p0, v, 0);
+                lf[v] = A.X.at(1, p1, v, -1);
+                rt[v] = A.X.at(1, p1, v, 1);
             }
             rhs_vals(m, lap_weights(m, cc), yc, top, y3, lf, rt, A.E3, f3);
 #pragma unroll
@@ -467,8 +505,8 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
     // this row's y(t_{i-1} + dt/3) and Z1 over the row two above (read above, own column)
 #pragma unroll
     for (int v = 0; v < 3; v++) {
-        A.X13.at(p0, v, c) = y3[v];
-        A.X1.at(p0, v, c) = Z1[v];
+        A.X.at(1, p0, v, 0) = y3[v];
+        A.X.at(2, p0, v, 0) = Z1[v];
     }
     // stage 3 on row t-2 with J(yh) of that row
     float Z3[3] = {0.0f, 0.0f, 0.0f};
@@ -478,8 +516,8 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         float lt[3], zc[3];
 #pragma unroll
         for (int v = 0; v < 3; v++) {
-            zc[v] = A.X2.at(p0, v, c);
-            lt[v] = m.lc[v] * stencil5(tw, A.X2.at(p1, v, c), Z2[v], A.X2.at(p0, v, c - 1), A.X2.at(p0, v, c + 1), zc[v]);
+            zc[v] = A.X.at(3, p0, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, A.X.at(3, p1, v, 0), Z2[v], A.X.at(3, p0, v, -1), A.X.at(3, p0, v, 1), zc[v]);
         }
         if (cc & C_ACTIVE) {
             float o[3];
@@ -494,7 +532,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
     }
     if (t >= A.tb + 2)
 #pragma unroll
-        for (int v = 0; v < 3; v++) A.X2.at(p1, v, c) = Z2[v];
+        for (int v = 0; v < 3; v++) A.X.at(3, p1, v, 0) = Z2[v];
     // stage 4 on row t-3 with J(y_{i-1}); the new z and the row's z.gh sum
     if (t >= A.tb + 6) {
         const int row = t - 3;
@@ -504,8 +542,8 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         float lt[3], zc[3], zo[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
         for (int v = 0; v < 3; v++) {
-            zc[v] = A.X3.at(p1, v, c);
-            lt[v] = m.lc[v] * stencil5(tw, A.X3.at(p0, v, c), Z3[v], A.X3.at(p1, v, c - 1), A.X3.at(p1, v, c + 1), zc[v]);
+            zc[v] = A.X.at(4, p1, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, A.X.at(4, p0, v, 0), Z3[v], A.X.at(4, p1, v, -1), A.X.at(4, p1, v, 1), zc[v]);
         }
         if (act) {
             Gates32 g = gates32(m, A.ar[0]);
@@ -515,8 +553,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
 #pragma unroll
             for (int v = 0; v < 3; v++) zo[v] = A.z[qm3][v] + (A.dz[qm3][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
         }
-#pragma unroll
-        for (int v = 0; v < 3; v++) st_row(A.zout + (size_t)v * A.n, row, P, c, zo[v]);
+        st3<P>(A.zout, row, A.n, A.c4, zo);
         const double rs = wave_sum64(zgh_term(m, act, cc, zo[0]));
         if ((c & 63) == 0) {
             const int j = (row - A.y0) % A.tyA;
@@ -528,7 +565,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
     }
     if (t >= A.tb + 4)
 #pragma unroll
-        for (int v = 0; v < 3; v++) A.X3.at(p0, v, c) = Z3[v];
+        for (int v = 0; v < 3; v++) A.X.at(4, p0, v, 0) = Z3[v];
     __syncthreads();
 }
 
@@ -554,15 +591,15 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
     const int buf = seg_buf(S, segi);
     const float *kbuf = S.kseg + (size_t)buf * S.K * st;
     const float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
-    A.yA = (i - 1 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 1 - seg_a) * st;
-    A.yB = (i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st;
-    A.kA = kbuf + (size_t)(i - 1 - seg_a) * st;
+    A.yA = buf_rsrc((i - 1 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 1 - seg_a) * st);
+    A.yB = buf_rsrc((i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st);
+    A.kA = buf_rsrc(kbuf + (size_t)(i - 1 - seg_a) * st);
     const float *knext = (b_end == S.nt) ? S.kT
                          : (segi + 1 >= S.keep0 ? S.kseg + (size_t)seg_buf(S, segi + 1) * S.K * st
                                                 : S.kfirst + (size_t)((segi + 1) % 3) * st);
-    A.kB = (i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext;
-    A.zin = S.zb + (size_t)((S.nt - i) & 1) * st;
-    A.zout = S.zb + (size_t)((S.nt - i + 1) & 1) * st;
+    A.kB = buf_rsrc((i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext);
+    A.zin = buf_rsrc(S.zb + (size_t)((S.nt - i) & 1) * st);
+    A.zout = buf_rsrc(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
     A.m = S.m;
     A.dt = S.dt;
     A.row_i = i - 1;
@@ -570,22 +607,22 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
     A.dEB = (i < S.nt) ? (float)(S.Ea[3 * A.row_i + 1] - S.Ef[3 * i]) : 0.0f;
     A.rows = S.rows;
     A.n = S.n;
-    A.code = S.code;
+    A.code = buf_rsrc(S.code);
     A.partial = S.partial;
     A.ntiles = S.ntiles;
     A.tyA = S.atile_rows;
     A.c = threadIdx.x;
+    A.c4 = (uint32_t)A.c * 4u;
+    A.c2 = (uint32_t)A.c * 2u;
     A.y0 = strip * Hs;
     A.y1 = min(A.y0 + Hs, S.rows);
     A.tb = A.y0 - 3;
     A.tacc = 0.0;
-    A.Xz.b = lds;
-    A.X13.b = lds + Ring2<P>::FLOATS;
-    A.X1.b = lds + 2 * Ring2<P>::FLOATS;
-    A.X2.b = lds + 3 * Ring2<P>::FLOATS;
-    A.X3.b = lds + 4 * Ring2<P>::FLOATS;
-    // zero pads: 5 rings x 2 rows x 3 planes, two pads each
-    for (int q = A.c; q < 60; q += P) lds[(q >> 1) * Ring2<P>::PP + ((q & 1) ? Ring2<P>::PP - 1 : 0)] = 0.0f;
+    A.X.b = lds + (A.c + 1) * Rings<P, 5>::CS;
+    pin_model(A.m);
+    // zero pads: columns -1 and P
+    constexpr int CSA = Rings<P, 5>::CS;
+    for (int q = A.c; q < 2 * CSA; q += P) lds[q < CSA ? q : (P + 1) * CSA + q - CSA] = 0.0f;
     // z of rows tb-1 and tb (row tb also for its neighbours' stencils), the arrivals of iteration tb
     A.load(A.tb - 2);   // z of row tb-1
 #pragma unroll
@@ -594,7 +631,7 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
 #pragma unroll
     for (int v = 0; v < 3; v++) {
         A.z[0][v] = A.zn[v];
-        A.Xz.at(0, v, A.c) = A.zn[v];
+        A.X.at(0, 0, v, 0) = A.zn[v];
     }
     A.load(A.tb);
     __syncthreads();
@@ -616,8 +653,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
 
 // ------------------------------------------------------------------------------ launchers
 // Dynamic LDS of the two kernels: 4 (forward) and 5 (adjoint) two-row rings of three planes.
-template <int NW>
-constexpr size_t roll_lds_bytes(int rings) { return sizeof(float) * (size_t)rings * Ring2<64 * NW>::FLOATS; }
+template <int NW, int NR>
+constexpr size_t roll_lds_bytes() { return sizeof(float) * (size_t)Rings<64 * NW, NR>::FLOATS; }
 
 template <int NW>
 void roll_launch_nw(bool adj, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, int Hs,
@@ -625,13 +662,13 @@ void roll_launch_nw(bool adj, const FusedSlot *slots_dev, const LaunchMap &map, 
 {
     const dim3 grid(launch_map_tiles(map)), block(64 * NW);
     if (adj) {
-        const size_t lds = roll_lds_bytes<NW>(5);
+        const size_t lds = roll_lds_bytes<NW, 5>();
         static const bool attr = hipFuncSetAttribute((const void *)k_roll_adj<NW>,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
         (void)attr;
         hipLaunchKernelGGL(k_roll_adj<NW>, grid, block, lds, stream, slots_dev, cur, map, Hs);
     } else {
-        const size_t lds = roll_lds_bytes<NW>(4);
+        const size_t lds = roll_lds_bytes<NW, 4>();
         static const bool attr = hipFuncSetAttribute((const void *)k_roll_fwd<NW>,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds) == hipSuccess;
         (void)attr;

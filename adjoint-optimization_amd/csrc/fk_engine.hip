@@ -16,6 +16,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <deque>
 #include <map>
@@ -598,8 +599,11 @@ int ctx_create(fkadj_instance *I, hipStream_t s, EvalCtx **out, bool self_group 
             ok = group_build(self, s, &C->group) == FKADJ_OK;
         }
     }
-    ok = ok && hipStreamSynchronize(s) == hipSuccess;
+    // an instance's own context is used from other streams too (batch lanes): wait for the zero
+    // fills; a server context is only used on the server's stream, where they are ordered already
+    if (self_group) ok = ok && hipStreamSynchronize(s) == hipSuccess;
     if (!ok) {
+        if (!self_group) (void)hipStreamSynchronize(s);
         ctx_free(C);
         return FKADJ_ERR_NOMEM;
     }
@@ -1887,10 +1891,28 @@ enum { RQ_QUEUED, RQ_RUNNING, RQ_FINISHING, RQ_DONE, RQ_CANCELLED, RQ_FAILED };
 enum { PH_FWD, PH_REC, PH_ADJ };
 constexpr int kRoundsInFlight = 8;
 
+// Evaluation contexts of server requests depend only on the instance's shape: a finished request's
+// context goes to a pool keyed by that shape and the next request of the same shape takes it (its
+// slot record is re-pointed at the new instance).  Pooled contexts count against the budget and are
+// freed when a request of another shape needs the room.
+struct CtxShape {
+    int prec, rows, pitch, nt, K, nb_part, nb_plain, n_num;
+    bool operator<(const CtxShape &o) const
+    {
+        const int a[8] = {prec, rows, pitch, nt, K, nb_part, nb_plain, n_num};
+        const int b[8] = {o.prec, o.rows, o.pitch, o.nt, o.K, o.nb_part, o.nb_plain, o.n_num};
+        return std::lexicographical_compare(a, a + 8, b, b + 8);
+    }
+};
+CtxShape shape_of(const fkadj_instance *I)
+{
+    return CtxShape{I->prec, I->rows, I->pitch, I->nt, I->K, I->nb_part, I->nb_plain, I->n_num};
+}
+
 struct ServReq {
     int64_t id = 0;
     fkadj_instance *I = nullptr;
-    EvalCtx *C = nullptr;        // this request's buffers, allocated at admission
+    EvalCtx *C = nullptr;        // this request's buffers, taken at admission (pooled by shape)
     std::vector<double> E;
     int state = RQ_QUEUED;
     int rc = FKADJ_OK;
@@ -1920,6 +1942,14 @@ struct fkadj_server {
     int64_t next_id = 1;
     hipEvent_t round_ev[kRoundsInFlight] = {};
     int64_t rounds = 0;
+    // free evaluation contexts by shape (their bytes are part of `used`), pinned result buffers by
+    // length, completion events
+    std::map<CtxShape, std::vector<EvalCtx *>> ctx_pool;
+    std::map<int, std::vector<double *>> out_pool;
+    std::vector<hipEvent_t> ev_pool;
+    // failure injection for tests: the request with this id reports a device error when its result
+    // is fetched, as a failed completion would (environment FKADJ_TEST_FAIL_REQUEST at creation)
+    int64_t fail_id = 0;
 };
 
 namespace {
@@ -1957,7 +1987,37 @@ int server_finish(fkadj_server *S, ServReq *r, bool cancelled)
     return FKADJ_OK;
 }
 
-// Completed requests: assemble results, free their contexts and return their budget.
+// Returns a finished request's context, pinned buffer and event to the pools (the context's bytes
+// stay counted in `used` while pooled).
+void server_release(fkadj_server *S, ServReq *r)
+{
+    if (r->C) S->ctx_pool[shape_of(r->I)].push_back(r->C);
+    r->C = nullptr;
+    if (r->h_out) S->out_pool[r->I->nt + 2].push_back(r->h_out);
+    r->h_out = nullptr;
+    if (r->done) S->ev_pool.push_back(r->done);
+    r->done = nullptr;
+}
+
+// Frees pooled contexts (largest first) until `need` more bytes fit the budget; false if they do not.
+bool server_make_room(fkadj_server *S, int64_t need)
+{
+    while (S->used + need > S->budget) {
+        EvalCtx *victim = nullptr;
+        std::vector<EvalCtx *> *from = nullptr;
+        for (auto &kv : S->ctx_pool)
+            for (auto *C : kv.second)
+                if (!victim || C->bytes > victim->bytes) { victim = C; from = &kv.second; }
+        if (!victim) return false;
+        from->erase(std::find(from->begin(), from->end(), victim));
+        S->used -= victim->bytes;
+        (void)hipStreamSynchronize(S->stream);   // its last request's kernels may still read it
+        ctx_free(victim);
+    }
+    return true;
+}
+
+// Completed requests: assemble results and return their context and buffers to the pools.
 int server_reap(fkadj_server *S, bool wait)
 {
     int n = 0;
@@ -1965,6 +2025,10 @@ int server_reap(fkadj_server *S, bool wait)
         ServReq *r = S->finishing[k];
         hipError_t q = wait ? hipEventSynchronize(r->done) : hipEventQuery(r->done);
         if (q == hipErrorNotReady) { k++; continue; }
+        if (q == hipSuccess && r->id == S->fail_id) {
+            q = hipErrorLaunchFailure;
+            snprintf(g_errmsg, sizeof g_errmsg, "injected completion failure of request %lld", (long long)r->id);
+        }
         if (q != hipSuccess && r->rc == FKADJ_OK) r->rc = FKADJ_ERR_DEVICE;
         fkadj_instance *I = r->I;
         if (r->rc == FKADJ_OK) {
@@ -1978,11 +2042,7 @@ int server_reap(fkadj_server *S, bool wait)
             }
         }
         r->state = r->rc == FKADJ_OK ? RQ_DONE : (r->rc == FKADJ_ERR_CANCELLED ? RQ_CANCELLED : RQ_FAILED);
-        if (r->h_out) (void)hipHostFree(r->h_out);
-        r->h_out = nullptr;
-        ctx_free(r->C);
-        r->C = nullptr;
-        S->used -= r->bytes;
+        server_release(S, r);
         I->busy--;
         S->finishing.erase(S->finishing.begin() + k);
         n++;
@@ -2009,31 +2069,53 @@ int server_admit(fkadj_server *S)
             S->queue.erase(S->queue.begin() + q);
             continue;
         }
-        if (S->used + bytes > S->budget) return FKADJ_OK;
-        int rc = ctx_create(I, S->stream, &r->C, false);
+        // a pooled context of this shape (already counted in `used`), else room for a new one
+        auto &pool = S->ctx_pool[shape_of(I)];
+        int rc = FKADJ_OK;
+        if (!pool.empty()) {
+            r->C = pool.back();
+            pool.pop_back();
+            r->C->I = I;
+            ctx_fill_slot(r->C);
+            // the objective writes the terminal adjoint at active cells only: hole cells of this
+            // instance must read 0, whatever the previous instance left there
+            if (hipMemsetAsync(r->C->d_z, 0, 2 * I->st_bytes, S->stream) != hipSuccess) rc = FKADJ_ERR_DEVICE;
+        } else {
+            if (!server_make_room(S, bytes)) return FKADJ_OK;   // wait for running requests to finish
+            rc = ctx_create(I, S->stream, &r->C, false);
+            if (rc == FKADJ_OK) S->used += bytes;
+        }
         if (rc == FKADJ_OK) rc = build_tables(I, r->C, r->E.data(), S->stream);
-        if (rc == FKADJ_OK && hipHostMalloc((void **)&r->h_out, sizeof(double) * (I->nt + 2)) != hipSuccess)
-            rc = FKADJ_ERR_NOMEM;
-        if (rc == FKADJ_OK && hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess)
-            rc = FKADJ_ERR_DEVICE;
+        if (rc == FKADJ_OK) {
+            auto &op = S->out_pool[I->nt + 2];
+            if (!op.empty()) {
+                r->h_out = op.back();
+                op.pop_back();
+            } else if (hipHostMalloc((void **)&r->h_out, sizeof(double) * (I->nt + 2)) != hipSuccess) {
+                rc = FKADJ_ERR_NOMEM;
+            }
+        }
+        if (rc == FKADJ_OK) {
+            if (!S->ev_pool.empty()) {
+                r->done = S->ev_pool.back();
+                S->ev_pool.pop_back();
+            } else if (hipEventCreateWithFlags(&r->done, hipEventDisableTiming) != hipSuccess) {
+                rc = FKADJ_ERR_DEVICE;
+            }
+        }
         if (rc == FKADJ_OK &&
             hipMemcpyAsync(I->state(r->C->d_ck, 0), I->d_y0, I->st_bytes, hipMemcpyDeviceToDevice, S->stream) != hipSuccess)
             rc = FKADJ_ERR_DEVICE;
         if (rc == FKADJ_OK) rc = server_upload_slot(S, free_slot, r->C->slot_h);
         S->queue.erase(S->queue.begin() + q);
+        r->bytes = bytes;
         if (rc != FKADJ_OK) {
             r->rc = rc;
             r->state = RQ_FAILED;
             I->busy--;
-            if (r->h_out) (void)hipHostFree(r->h_out);
-            r->h_out = nullptr;
-            if (r->C) (void)hipStreamSynchronize(S->stream);
-            ctx_free(r->C);
-            r->C = nullptr;
+            server_release(S, r);   // a context it took stays pooled (and counted)
             continue;
         }
-        S->used += bytes;
-        r->bytes = bytes;
         r->slot = free_slot;
         r->state = RQ_RUNNING;
         r->phase = PH_FWD;
@@ -2125,6 +2207,7 @@ int fkadj_server_create(int64_t budget_bytes, int max_slots, fkadj_server **out)
     if (!S) return FKADJ_ERR_NOMEM;
     S->budget = budget_bytes;
     S->max_slots = max_slots;
+    if (const char *f = getenv("FKADJ_TEST_FAIL_REQUEST")) S->fail_id = atoll(f);
     S->slot_req.assign(max_slots, nullptr);
     bool ok = hipStreamCreateWithFlags(&S->stream, hipStreamNonBlocking) == hipSuccess &&
               hipMalloc((void **)&S->d_slots, sizeof(FusedSlot) * max_slots) == hipSuccess &&
@@ -2158,6 +2241,11 @@ int fkadj_server_destroy(fkadj_server *S)
         if (r->done) (void)hipEventDestroy(r->done);
         delete r;
     }
+    for (auto &kv : S->ctx_pool)
+        for (auto *C : kv.second) ctx_free(C);
+    for (auto &kv : S->out_pool)
+        for (auto *h : kv.second) (void)hipHostFree(h);
+    for (auto e : S->ev_pool) (void)hipEventDestroy(e);
     if (S->d_slots) (void)hipFree(S->d_slots);
     if (S->h_slots) (void)hipHostFree(S->h_slots);
     for (int k = 0; k < kRoundsInFlight; k++)

@@ -179,3 +179,86 @@ def test_requests_on_instances_with_kept_segments_equal_alone():
         assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
     for inst, _ in pairs:
         inst.close()
+
+
+def test_pooled_context_reused_by_another_instance_of_the_same_shape():
+    """A finished request's buffers serve the next request of the same shape, on another instance:
+    a mirrored hole layout (same active cells, holes elsewhere) must not see the first instance's
+    values at its hole cells.  Each result equals the instance alone."""
+    _print_case("server_pool_reuse")
+    m = fkadj.reference_model(h=0.025)
+    case = disc_case(96, 128, 7, 31)
+    masks = [case.mask, case.mask[:, ::-1].copy()]
+    pairs = []
+    for k, mk in enumerate(masks):
+        ai = fkadj.geometry.active_index_from_mask(mk)
+        y = np.concatenate([case.y0[v * case.n_act:(v + 1) * case.n_act] for v in range(3)])
+        inst = fkadj.Instance(ai, 96, 128, dt=0.02, nt=80, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=8,
+                              alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=20,
+                              lazy_workspace=True)
+        inst.set_state(y)
+        pairs.append((inst, np.random.default_rng(40 + k).uniform(0, 10, 8)))
+    alone = [inst.grad(A) for inst, A in pairs]
+    with fkadj.Server(budget_bytes=pairs[0][0].request_bytes, max_slots=2) as srv:
+        out = []
+        for inst, A in pairs + pairs[:1]:             # one at a time: each takes the pooled buffers
+            rid = srv.submit(inst, A)
+            while srv.pending():
+                srv.run(0)
+            out.append(srv.poll(rid))
+    for a, r in zip(alone + alone[:1], out):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1]
+    for inst, _ in pairs:
+        inst.close()
+
+
+def test_injected_device_error_fails_only_its_request(monkeypatch):
+    """A completion failure forced from the host on one request of a running mix (no device fault is
+    involved): that request reports FKADJ_ERR_DEVICE, every other request equals its instance alone."""
+    _print_case("server_fault_injection")
+    pairs = _instances(SPECS)
+    alone = [inst.grad(A) for inst, A in pairs]
+    monkeypatch.setenv("FKADJ_TEST_FAIL_REQUEST", "3")   # ids count from 1 per server
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=8) as srv:
+        ids = [srv.submit(inst, A) for inst, A in pairs]
+        assert ids[2] == 3
+        while srv.pending():
+            srv.run(0)
+        for k, rid in enumerate(ids):
+            if k == 2:
+                with pytest.raises(fkadj.FkadjError) as e:
+                    srv.poll(rid)
+                assert e.value.code == -4 and "injected" in str(e.value)
+            else:
+                r = srv.poll(rid)
+                assert np.array_equal(alone[k][0], r[0]) and alone[k][1] == r[1]
+    for inst, _ in pairs:
+        inst.close()
+
+
+def test_nonfinite_request_fails_only_itself():
+    """A field large enough to drive its request's FP32 state to Inf/NaN: that request reports
+    FKADJ_ERR_NONFINITE, the requests sharing its launches equal their instances alone, and the same
+    instance serves a normal field afterwards."""
+    _print_case("server_nonfinite")
+    pairs = _instances(SPECS[:3])
+    alone = [inst.grad(A) for inst, A in pairs]
+    huge = np.full_like(pairs[1][1], 1e30)
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=8) as srv:
+        ids = [srv.submit(inst, A) for inst, A in pairs]
+        bad = srv.submit(pairs[1][0], huge)
+        while srv.pending():
+            srv.run(0)
+        with pytest.raises(fkadj.FkadjError) as e:
+            srv.poll(bad)
+        assert e.value.code == -5
+        for k, rid in enumerate(ids):
+            r = srv.poll(rid)
+            assert np.array_equal(alone[k][0], r[0]) and alone[k][1] == r[1]
+        rid = srv.submit(*pairs[1])                  # the pooled buffers of the failed request again
+        while srv.pending():
+            srv.run(0)
+        r = srv.poll(rid)
+        assert np.array_equal(alone[1][0], r[0])
+    for inst, _ in pairs:
+        inst.close()

@@ -2138,10 +2138,26 @@ int server_round(fkadj_server *S)
         launch_map_clear(rm[c]);
     }
     for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
-    int64_t fcells = 0;   // cells of this round's forward-kind launch
+    int64_t fcells = 0, acells = 0;   // cells of this round's forward-kind and adjoint launches
     for (int k = 0; k < S->max_slots; k++)
-        if (S->slot_req[k] && S->slot_req[k]->phase != PH_ADJ)
-            fcells += (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
+        if (S->slot_req[k]) {
+            const int64_t cells = (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
+            (S->slot_req[k]->phase != PH_ADJ ? fcells : acells) += cells;
+        }
+    // large rounds take the row-rolling kernels for the slots whose pitch they support (one launch
+    // per pitch; the same results as the tiled kernels)
+    const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
+    const bool rollf = rmode == 2 || (rmode == 1 && fcells >= kRollCells);
+    const bool rolla = rmode == 2 || (rmode == 1 && acells >= kRollCells);
+    std::map<int, LaunchMap> rfm, ram;
+    auto rmap = [](std::map<int, LaunchMap> &mm, int pitch) -> LaunchMap & {
+        auto it = mm.find(pitch);
+        if (it == mm.end()) {
+            it = mm.emplace(pitch, LaunchMap{}).first;
+            launch_map_clear(it->second);
+        }
+        return it->second;
+    };
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];
         if (!r) continue;
@@ -2149,15 +2165,23 @@ int server_round(fkadj_server *S)
         const int K = I->K, f = fwd_cfg_of(I, fcells);
         if (r->phase == PH_ADJ) {
             ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
-            launch_map_add(am[I->adj_cfg], k, tiles_of(I->rows, I->pitch, I->adj_cfg));
+            if (rolla && roll_pitch_supported(I->pitch) && hs % tile_rows_of(I->adj_cfg) == 0)
+                launch_map_add(rmap(ram, I->pitch), k, (I->rows + hs - 1) / hs);
+            else
+                launch_map_add(am[I->adj_cfg], k, tiles_of(I->rows, I->pitch, I->adj_cfg));
         } else {
             fc.c[k] = r->phase == PH_FWD ? SlotCursor{MODE_FWD, r->step, 0, 0}
                                          : SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0};
-            launch_map_add(fm[f], k, tiles_of(I->rows, I->pitch, f));
+            if (rollf && roll_pitch_supported(I->pitch))
+                launch_map_add(rmap(rfm, I->pitch), k, (I->rows + hs - 1) / hs);
+            else
+                launch_map_add(fm[f], k, tiles_of(I->rows, I->pitch, f));
         }
     }
     for (int c = 1; c < NUM_TILE_CFG; c++) fused_forward_launch(c, S->d_slots, fm[c], fc, S->stream);
+    for (auto &kv : rfm) roll_launch(false, kv.first, S->d_slots, kv.second, fc, hs, S->stream);
     for (int c = 1; c < NUM_TILE_CFG; c++) fused_adjoint_launch(c, S->d_slots, am[c], ac, S->stream);
+    for (auto &kv : ram) roll_launch(true, kv.first, S->d_slots, kv.second, ac, hs, S->stream);
     HIPCHK(hipGetLastError());
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];

@@ -92,3 +92,25 @@ def test_rolling_batch_of_512_instances_equals_alone():
     _same(alone, batched)
     for i in insts:
         i.close()
+
+
+def test_served_requests_on_rolling_rounds_equal_alone():
+    """Server rounds with the rolling kernels forced: requests of pitch 256, 384 and 512 at different
+    steps and phases share each rolling launch (one per pitch), join mid-flight, and every result
+    equals its instance alone on the tiled kernels."""
+    _print_case("roll_served", n=4)
+    specs = [(40, 250, 40, 4, 10, 61), (56, 500, 60, 6, 20, 62), (33, 380, 40, 4, 10, 63), (48, 250, 60, 6, 15, 64)]
+    pairs = [_instance(H, W, nt, K, ck, seed) for (H, W, nt, K, ck, seed) in specs]
+    fkadj.set_rolling(0, 16)
+    alone = [inst.grad(E) for inst, E in pairs]
+    fkadj.set_rolling(2, 16)
+    with fkadj.Server(budget_bytes=1 << 34, max_slots=8) as srv:
+        ids = [srv.submit(*pairs[0]), srv.submit(*pairs[1])]
+        srv.run(23)
+        ids += [srv.submit(*pairs[2]), srv.submit(*pairs[3])]
+        while srv.pending():
+            srv.run(0)
+        res = [srv.poll(i) for i in ids]
+    _same(alone, res)
+    for inst, _ in pairs:
+        inst.close()

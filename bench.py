@@ -8,6 +8,12 @@ steps with recomputation, objective mean(u(T)^2) + 1e-3 * pulse energy, S1-S2 sp
 state; every instance has its own seed.  A step is one batched gradient evaluation of all
 instances.  Inputs are resident in HBM before the timed region.
 
+The default run reports, in one JSON line: the headline (16 instances batched, spare HBM keeping
+their last trajectory segments), the strict schedule (every segment recomputed) and its speedup,
+the plain FP64 path, the FP32 error, the roofline of the dominant kernel (HIP events in the run,
+traffic from the committed counter profile), the measured HBM copy rate, a short served stream
+with p50/p99 latency, and the FP64 oracle's full evaluation on the host cores.
+
 Prints one JSON line on stdout; progress goes to stderr.
 """
 from __future__ import annotations
@@ -24,14 +30,15 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per cell of one launch of the fused kernels (forward and adjoint on 64x16/512 tiles, the
-# benchmark's launch shapes), measured with rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE on a 16 x 512^2
-# batch and scaled by the factors a 1 GiB copy kernel gives those counters (x2.0 and x1.0); forward is
-# the mean of forward-sweep and recompute launches
-HBM_BYTES_PER_CELL = {"adj": 76.2, "fwd": 32.8}
-# VALU lane-instructions per cell of one launch, same launch shapes and runs (SQ_INSTS_VALU x 64 / cells);
+# Measured per-cell figures of the kernels of the benchmark's launches (16 x 512^2 split into two groups
+# of 8, 2 097 152 cells per dispatch), from rocprofv3 --pmc runs summarised in
+# profiles/roll_c2x16_counters.md: HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors a 2 GiB
+# copy gives those counters), VALU lane-instructions = SQ_INSTS_VALU x 64 / cells.  The forward figure
+# is the mean of the forward-sweep and recompute launches.
+PROFILE = "profiles/roll_c2x16_counters.md"
+HBM_BYTES_PER_CELL = {"adj": 111.4, "fwd": 40.0}
+VALU_LANE_INSTR_PER_CELL = {"adj": 675.0, "fwd": 378.0}
 # the issue peak is one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz (non-packed FP32)
-VALU_LANE_INSTR_PER_CELL = {"adj": 820.0, "fwd": 512.0}
 VALU_PEAK_G_WAVE_INSTR_S = 256 * 2.4
 
 
@@ -57,10 +64,34 @@ def _cpu_model() -> str:
     return "unknown"
 
 
-def cpu_baseline(mask, y0, A, nt_full, nt_sample=100, nt_single=10):
-    """FP64 oracle (C, OpenMP, -O3 AVX2) on the host: a bounded sample of the same workload (the
-    first nt_sample steps of instance 0, all three sweeps) on all host threads, scaled to a full
-    evaluation; and the first nt_single steps on one thread."""
+def copy_peak(gib: float = 2.0, reps: int = 10) -> dict:
+    """Measured HBM copy rate: a plain device-to-device copy of `gib` GiB (larger than the 256 MB
+    Infinity Cache), timed with events over `reps` copies; bytes = read + written."""
+    import torch
+    n = int(gib * (1 << 30)) // 4
+    a = torch.ones(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        b.copy_(a)
+    e1.record()
+    torch.cuda.synchronize()
+    ms = e0.elapsed_time(e1) / reps
+    del a, b
+    torch.cuda.empty_cache()
+    return {"GBs": 2 * n * 4 / (ms * 1e-3) / 1e9, "bytes_per_copy": 2 * n * 4, "ms_per_copy": ms,
+            "kernel": "torch copy_ (device to device), 2 GiB read + 2 GiB written"}
+
+
+def cpu_baseline(mask, y0, A, nt_full, nt_sample=None, nt_single=10):
+    """FP64 oracle (C, OpenMP, -O3 AVX2) on the host: one full evaluation of instance 0 (all
+    nt_full steps of the forward, recompute and adjoint sweeps; nt_sample < nt_full times only the
+    first steps and scales) on all host threads; and the first nt_single steps on one thread,
+    scaled (a labelled sample)."""
+    nt_sample = nt_full if nt_sample is None else nt_sample
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle as O
     from fkadj import geometry as G
@@ -86,9 +117,12 @@ def cpu_baseline(mask, y0, A, nt_full, nt_sample=100, nt_single=10):
     dt1 = timed(nt_single)
     O.set_threads(threads)
     per_eval = dt * nt_full / nt_sample
+    what = (f"one full evaluation of instance 0 (512^2, {nt_full} steps: forward, recompute and adjoint "
+            f"sweeps, {dt:.1f} s)" if nt_sample == nt_full else
+            f"instance 0 (512^2), first {nt_sample} of {nt_full} steps with forward, recompute and adjoint "
+            f"sweeps ({dt:.2f} s), scaled x{nt_full // nt_sample}")
     return {"value": 1.0 / per_eval, "unit": "evals/s", "cores": threads, "kind": "port",
-            "sample": f"FP64 oracle, instance 0 (512^2), first {nt_sample} of {nt_full} steps with forward, "
-                      f"recompute and adjoint sweeps ({dt:.2f} s), scaled x{nt_full // nt_sample}",
+            "sample": f"FP64 oracle, {what}",
             "seconds_per_eval": per_eval, "cpu_model": _cpu_model(), "nproc": os.cpu_count(),
             "single_thread": {"seconds_per_eval": dt1 * nt_full / nt_single,
                               "sample": f"first {nt_single} steps ({dt1:.2f} s) on 1 thread"}}
@@ -149,9 +183,9 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
     fwd_bytes = 7 * 4 * cells
     adj_bytes = 13 * 4 * cells
     if adj_ms * na >= fwd_ms * nf:
-        dom, dbytes, dms, dkey = "k_fused_adj (adjoint step)", adj_bytes, adj_ms, "adj"
+        dom, dbytes, dms, dkey = "k_roll_adj<8> (adjoint step, row-rolling)", adj_bytes, adj_ms, "adj"
     else:
-        dom, dbytes, dms, dkey = "k_fused_fwd (forward/recompute step)", fwd_bytes, fwd_ms, "fwd"
+        dom, dbytes, dms, dkey = "k_roll_fwd<8> (forward/recompute step, row-rolling)", fwd_bytes, fwd_ms, "fwd"
     achieved = dbytes / (dms * 1e-3) / 1e9
     # compulsory words of the launches of one evaluation (kept segments are not recomputed, so the
     # forward launches count nt sweep steps plus the recomputed ones)
@@ -162,7 +196,9 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
         return {"lane_instr_per_cell": VALU_LANE_INSTR_PER_CELL[key], "achieved": rate,
                 "peak": VALU_PEAK_G_WAVE_INSTR_S, "unit": "G wave-instr/s", "frac": rate / VALU_PEAK_G_WAVE_INSTR_S}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": HBM_BYTES_PER_CELL[dkey] * cells, "kernel": dom,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": HBM_BYTES_PER_CELL[dkey] * cells,
+            "traffic_GBs": HBM_BYTES_PER_CELL[dkey] * cells / (dms * 1e-3) / 1e9, "traffic_source": PROFILE,
+            "kernel": dom,
             "kernel_ms": dms, "bytes_per_launch": dbytes,
             "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
             "adj": {"ms": adj_ms, "GBs": adj_bytes / (adj_ms * 1e-3) / 1e9, "launches": na},
@@ -504,7 +540,15 @@ def main():
     ap.add_argument("--c4-instances", type=int, default=24)
     ap.add_argument("--c4-requests", type=int, default=96)
     ap.add_argument("--c4-rate", type=float, default=200.0, help="offered load of the C4 stream (req/s)")
+    ap.add_argument("--skip-copy", action="store_true")
+    ap.add_argument("--skip-served", action="store_true")
+    ap.add_argument("--served-rate", type=float, default=8.0, help="offered load of the served leg (req/s)")
+    ap.add_argument("--copy-peak", action="store_true",
+                    help="only measure the HBM copy rate (the counter calibration target) and exit")
     args = ap.parse_args()
+    if args.copy_peak:
+        print(json.dumps({"copy_peak": copy_peak()}), flush=True)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the batched instances of this "
               "workload are independent and would shard by instance with no communication.", file=sys.stderr)
@@ -521,6 +565,9 @@ def main():
     import fkadj
     from fkadj import workloads as W
 
+    copy = None if args.skip_copy else copy_peak()
+    if copy:
+        log(f"[bench] HBM copy peak (2 GiB device copy): {copy['GBs']:.0f} GB/s")
     cfg = W.CONFIGS["C2"]
     N, holes, K_amp, ckpt = args.grid, cfg["holes"], cfg["K"], cfg["ckpt"]
     T = args.T
@@ -536,6 +583,31 @@ def main():
     from fkadj import autotune
     _, adj_tile = autotune.lookup(N)   # the forward tile follows each launch's size (built-in rule)
     fwd_tile = 0
+
+    def make_batch(keep_bytes):
+        out = []
+        for m, y in zip(masks, states):
+            inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile,
+                                   keep_bytes=keep_bytes)
+            inst.set_state(y)
+            out.append(inst)
+        return out
+
+    # strict schedule first (every segment recomputed, checkpoints every 50 steps as C2 states),
+    # in the same run and with the same launch settings as the headline
+    strict = None
+    if not args.no_keep:
+        sinsts = make_batch(0)
+        fkadj.batch_grad(sinsts, A)
+        ts = time.perf_counter()
+        res_strict = fkadj.batch_grad(sinsts, A)
+        strict_s = time.perf_counter() - ts
+        strict = {"value": B / strict_s, "unit": "evals/s", "ms_per_step": 1e3 * strict_s, "kept_segments": 0,
+                  "schedule": f"every {ckpt}-step segment recomputed in the backward pass"}
+        for inst in sinsts:
+            inst.close()
+        log(f"[bench] strict schedule (no kept segments): {1e3 * strict_s:.1f} ms per batched evaluation")
+
     # spare HBM keeps the last trajectory segments of every instance (no recompute for them): what
     # is left after the batch's own buffers, a single-instance FP32 and an FP64 instance, and 16 GB
     fp32 = footprint(N, N, nt, ckpt, fkadj.F32, adj_tile)
@@ -546,12 +618,7 @@ def main():
     keep = keeps[0]
     log(f"[bench] {free0 / 1e9:.0f} of {total0 / 1e9:.0f} GB free; keeping {kept} of {(nt + ckpt - 1) // ckpt} "
         f"segments per instance ({keep / 1e9:.1f} GB each)")
-    insts = []
-    for m, y in zip(masks, states):
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile,
-                               keep_bytes=keep)
-        inst.set_state(y)
-        insts.append(inst)
+    insts = make_batch(keep)
 
     # warmup + timed batched evaluations: the library runs the batch as two groups of 8, each from
     # its own host thread on its own stream (fkadj_set_batch_groups), with no per-launch events
@@ -581,7 +648,8 @@ def main():
 
     # instance 0 alone (its kept segments too), then single-instance FP32 latency on an instance
     # that keeps no segments (a lone 512^2 instance overlaps its recompute with the adjoint on a
-    # second stream, so keeping gains it nothing): all three results are the same bit for bit
+    # second stream, so keeping gains it nothing): all results are the same bit for bit, also the
+    # strict schedule's
     g_alone = insts[0].grad(A[0])
     ish = W.make_instance(masks[0], T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)
     ish.set_state(states[0])
@@ -591,9 +659,12 @@ def main():
         g32_single = ish.grad(A[0])
         lat.append(1e3 * (time.perf_counter() - ts))
     ish.close()
-    bit_identical = bool(all(np.array_equal(g[0], res[0][0]) and g[1] == res[0][1] for g in (g_alone, g32_single)) and
+    same = [(g_alone[0], g_alone[1]), (g32_single[0], g32_single[1])]
+    if strict:
+        same.append((res_strict[0][0], res_strict[0][1]))
+    bit_identical = bool(all(np.array_equal(g, res[0][0]) and L == res[0][1] for g, L in same) and
                          all(np.array_equal(a[0], b[0]) and a[1] == b[1] for a, b in zip(res, res_one)))
-    log(f"[bench] single FP32 eval {np.median(lat):.1f} ms; alone == batched: {bit_identical}")
+    log(f"[bench] single FP32 eval {np.median(lat):.1f} ms; alone == batched == strict: {bit_identical}")
 
     # plain FP64 single instance: the baseline path and the FP64 reference of instance 0
     plain = None
@@ -612,12 +683,22 @@ def main():
                "grad_rel_max": float(np.max(np.abs(g32 - g64)) / np.max(np.abs(g64))),
                "J_rel": float(abs(Lc32 - Lc64) / abs(Lc64)), "J_f64": Lc64, "J_f32": Lc32}
         log(f"[bench] plain FP64 {1e3 * t64:.0f} ms/eval; rel err grad {rel['grad_rel_l2']:.2e} J {rel['J_rel']:.2e}")
+    for inst in insts:
+        inst.close()
 
     # roofline of the dominant kernel from the in-run HIP-event samples
-    cells = sum(inst.nrows * ((inst.ncols + 63) // 64 * 64) for inst in insts)
+    cells = B * N * ((N + 63) // 64 * 64)
     roof = roofline_of(timings, cells, ckpt, nt, args.steps, total)   # cells: all instances of a step
     roof["source"] = (f"one extra batched evaluation run as one group with HIP events on one launch in eight "
                       f"({roof_ms:.0f} ms); the timed steps run {groups} groups without events")
+    if copy:
+        roof["copy_peak_GBs"] = copy["GBs"]
+        roof["frac_of_copy_peak"] = roof["achieved"] / copy["GBs"]
+
+    served = None if args.skip_served else served_leg(masks, states, A, T, K_amp, ckpt, adj_tile, args.served_rate)
+    if served:
+        log(f"[bench] served: {served['achieved_req_s']:.2f} req/s, p50 {served['latency_ms']['p50']:.0f} ms, "
+            f"p99 {served['latency_ms']['p99']:.0f} ms")
 
     cpu = None if args.skip_cpu else cpu_baseline(masks[0], states[0], A[0], nt)
     if cpu:
@@ -634,18 +715,70 @@ def main():
                    "kept_segments": int(kept), "keep_bytes_per_instance": int(keep), "batch_groups": groups},
         "plain_f64": plain,
         "speedup_vs_plain": (value / plain["value"]) if plain else None,
+        "strict_schedule": strict,
+        "strict_speedup_vs_plain": (strict["value"] / plain["value"]) if (plain and strict) else None,
         "rel_err_vs_f64": rel,
         "alone_equals_batched": bit_identical,
         "latency_ms": {"batched_step_p50": float(np.percentile(step_ms, 50)),
                        "batched_step_p99": float(np.percentile(step_ms, 99)),
                        "single_f32_p50": float(np.percentile(lat, 50)),
-                       "single_f32_p99": float(np.percentile(lat, 99))},
+                       "single_f32_p99": float(np.percentile(lat, 99)),
+                       "served_p50": served["latency_ms"]["p50"] if served else None,
+                       "served_p99": served["latency_ms"]["p99"] if served else None},
+        "served": served,
         "roofline": roof,
+        "copy_peak": copy,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)
+
+
+def served_leg(masks, states, A, T, K_amp, ckpt, adj_tile, rate, seed=7):
+    """A short served stream: one request per C2 instance (resident geometry and state, lazy
+    buffers), Poisson arrivals at `rate` req/s, served by the continuous-batching server (budget
+    200 GB, 64 slots); latency is arrival to result."""
+    import fkadj
+    from fkadj import workloads as W
+    insts = []
+    for m, y in zip(masks, states):
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, adj_tile=adj_tile, lazy_workspace=True)
+        inst.set_state(y)
+        insts.append(inst)
+    R = len(insts)
+    rng = np.random.default_rng(seed)
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    srv = fkadj.Server(budget_bytes=200 * 10 ** 9, max_slots=64)
+    ids, done_t, results = [None] * R, [None] * R, {}
+    nxt = 0
+    start = time.perf_counter()
+    while len(results) < R:
+        now = time.perf_counter() - start
+        while nxt < R and arrivals[nxt] <= now:
+            ids[nxt] = srv.submit(insts[nxt], A[nxt])
+            nxt += 1
+        if srv.pending() == 0:
+            time.sleep(max(0.0, min(0.001, arrivals[nxt] - now)) if nxt < R else 0)
+            continue
+        if srv.run(1) > 0:
+            for r in range(nxt):
+                if r in results:
+                    continue
+                got = srv.poll(ids[r])
+                if got is not None:
+                    results[r] = got
+                    done_t[r] = time.perf_counter() - start
+    makespan = max(done_t)
+    srv.close()
+    ref0 = insts[0].grad(A[0])
     for inst in insts:
         inst.close()
+    lat = np.array([1e3 * (done_t[r] - arrivals[r]) for r in range(R)])
+    return {"requests": R, "offered_req_s": rate, "achieved_req_s": R / makespan, "makespan_s": makespan,
+            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                           "min": float(lat.min()), "max": float(lat.max())},
+            "request0_equals_alone": bool(np.array_equal(results[0][0], ref0[0]) and results[0][1] == ref0[1]),
+            "workload": f"one request per C2 instance ({R} x 512^2, T={T:g} ms), Poisson {rate:g} req/s, "
+                        "budget 200 GB"}
 
 
 if __name__ == "__main__":

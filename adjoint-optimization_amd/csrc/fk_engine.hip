@@ -275,7 +275,11 @@ void fill_models(fkadj_instance *I)
 // The forward tile does not change any result, so it follows the launch's size.  The adjoint tile
 // regroups the z.gh partial sums, so it is fixed per instance (by its own size): batched or alone,
 // an instance gives the same result.
-int default_adj_cfg(int rows, int pitch) { return (int64_t)rows * pitch <= 65536 ? TILE_8_512 : TILE_16_512; }
+// The adjoint tile only regroups work: every adjoint kernel writes one z.gh partial per 64 x 8 cell block
+// (PART_ROWS), so it follows the launch's size as the forward tile does.
+// Measured: a lone 256^2 evaluation (T = 100 ms) takes 77 ms with 64x8 tiles and 91 ms with 64x16, a lone
+// 512^2 one (T = 200 ms) 234 and 230 ms.
+int default_adj_cfg(int64_t launch_cells) { return launch_cells <= ((int64_t)1 << 17) ? TILE_8_512 : TILE_16_512; }
 int default_fwd_cfg(int64_t launch_cells) { return launch_cells < ((int64_t)1 << 20) ? TILE_8_512 : TILE_16_512; }
 
 int validate_and_codes(const fkadj_problem *p, int pitch, std::vector<uint16_t> &code, std::vector<int32_t> &cell_of,
@@ -554,7 +558,6 @@ void ctx_fill_slot(EvalCtx *C)
     S.rows = I->rows; S.pitch = I->pitch; S.n = I->n;
     S.nt = I->nt; S.K = I->K; S.nck = I->nck;
     S.ntiles = I->nb_part;
-    S.atile_rows = tile_rows_of(I->adj_cfg);
     S.seg_par = C->seg_copies - 1;
     S.keep0 = (I->nt + I->K - 1) / I->K - C->keep;
     S.code = I->d_code;
@@ -693,6 +696,10 @@ int fwd_cfg_of(const fkadj_instance *I, int64_t launch_cells)
 {
     return I->fwd_cfg > 0 ? I->fwd_cfg : default_fwd_cfg(launch_cells);
 }
+int adj_cfg_of(const fkadj_instance *I, int64_t launch_cells)
+{
+    return I->adj_cfg > 0 ? I->adj_cfg : default_adj_cfg(launch_cells);
+}
 
 // Row-rolling kernels: 0 never, 1 for groups of 2M cells or more (default), 2 for every group that
 // can use them; strip rows (a multiple of 8 and of every member's adjoint tile rows).
@@ -725,7 +732,8 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         g->keep0 = std::max(g->keep0, members[k]->slot_h.keep0);
         const int fc = fwd_cfg_of(I, cells);
         launch_map_add(g->fwd[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
-        launch_map_add(g->adj[I->adj_cfg], (int)k, tiles_of(I->rows, I->pitch, I->adj_cfg));
+        const int ac = adj_cfg_of(I, cells);
+        launch_map_add(g->adj[ac], (int)k, tiles_of(I->rows, I->pitch, ac));
         launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
         g->depth2 = g->depth2 && I->fwd_cfg == 0;
     }
@@ -734,7 +742,7 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
     bool roll = rmode == 2 || (rmode == 1 && cells >= kRollCells);
     for (size_t k = 0; k < members.size() && roll; k++) {
         const fkadj_instance *I = members[k]->I;
-        roll = I->pitch == I0->pitch && roll_pitch_supported(I->pitch) && hs % tile_rows_of(I->adj_cfg) == 0;
+        roll = I->pitch == I0->pitch && roll_pitch_supported(I->pitch);
     }
     g->roll = roll;
     g->roll_pitch = I0->pitch;
@@ -1069,8 +1077,7 @@ int64_t fkadj_footprint_bytes(const fkadj_problem *p)
     const int64_t nck = (p->nt + K - 1) / K + 1;
     const int64_t states = 1 + ctx_states(p->precision, nck, K, seg_copies_of(p->precision, n, true),
                                           kept_segments(p->precision, p->keep_bytes, st, K, p->nt));
-    const int acfg = p->adj_tile > 0 && p->adj_tile < NUM_TILE_CFG ? p->adj_tile : default_adj_cfg(p->nrows, pitch);
-    int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : tiles_of(p->nrows, pitch, acfg);
+    int64_t nb = p->precision == FKADJ_F64 ? (pitch / 64) * ((p->nrows + 3) / 4) : (pitch / 64) * ((p->nrows + PART_ROWS - 1) / PART_ROWS);
     int64_t extra = n * 3 /*code+mask*/ + 16 * (int64_t)p->nt * 3 + 8 * (int64_t)p->nt * nb + 8 * (p->nt + 1) +
                     8 * 3 * n;
     return states * st + extra;
@@ -1184,9 +1191,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     I->nb_plain = (I->pitch / PLAIN_BX) * ((I->rows + PLAIN_BY - 1) / PLAIN_BY);
     if (I->prec == FKADJ_F32) {
         I->fwd_cfg = p->fwd_tile;   // 0: chosen per group from its total size
-        I->adj_cfg = p->adj_tile > 0 ? p->adj_tile : default_adj_cfg(I->rows, I->pitch);
-        const int ty_a = tile_rows_of(I->adj_cfg);
-        I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + ty_a - 1) / ty_a);   // adjoint tiles
+        I->adj_cfg = p->adj_tile;   // 0: chosen per launch from its total size
+        I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + PART_ROWS - 1) / PART_ROWS);   // 64 x 8 partial blocks
     } else {
         I->nb_part = I->nb_plain;
     }
@@ -2165,10 +2171,11 @@ int server_round(fkadj_server *S)
         const int K = I->K, f = fwd_cfg_of(I, fcells);
         if (r->phase == PH_ADJ) {
             ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
-            if (rolla && roll_pitch_supported(I->pitch) && hs % tile_rows_of(I->adj_cfg) == 0)
+            const int ac = adj_cfg_of(I, acells);
+            if (rolla && roll_pitch_supported(I->pitch))
                 launch_map_add(rmap(ram, I->pitch), k, (I->rows + hs - 1) / hs);
             else
-                launch_map_add(am[I->adj_cfg], k, tiles_of(I->rows, I->pitch, I->adj_cfg));
+                launch_map_add(am[ac], k, tiles_of(I->rows, I->pitch, ac));
         } else {
             fc.c[k] = r->phase == PH_FWD ? SlotCursor{MODE_FWD, r->step, 0, 0}
                                          : SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0};

@@ -90,6 +90,9 @@ void plain_zgh(const PlainCtx<T, M> &c, const T *z, double *partial_row);
 // ---- fused FP32 path: one launch per time step over the tiles of one or more instances
 constexpr int FUSED_TX = 64;    // tile width: one wavefront row
 constexpr int FUSED_HALO = 4;   // one cell per RK stage
+// Rows of the cell blocks that own one z.gh partial each (64 x PART_ROWS cells, numbered row-major): every
+// adjoint kernel adds a block's per-row wavefront sums in row order from 0.0, whatever its tile.
+constexpr int PART_ROWS = 8;
 // tile configurations (rows x threads): 1 = 16 x 256, 2 = 32 x 256, 3 = 32 x 512, 4 = 16 x 512, 5 = 32 x 1024,
 // 6 = 8 x 512, 7 = 8 x 256
 constexpr int TILE_16_256 = 1, TILE_32_256 = 2, TILE_32_512 = 3, TILE_16_512 = 4, TILE_32_1024 = 5, TILE_8_512 = 6,
@@ -103,8 +106,7 @@ __host__ __device__ constexpr int tile_rows_of(int cfg)
 struct FusedSlot {
     int rows, pitch, n;
     int nt, K, nck;             // steps, checkpoint interval, checkpoints
-    int ntiles;                 // tiles of this instance (zgh partial row length)
-    int atile_rows;             // rows of the instance's adjoint tiles (64 x atile_rows cells per partial)
+    int ntiles;                 // z.gh partial blocks of this instance (64 x PART_ROWS cells each; partial row length)
     int seg_par;                // 1: seg and kseg hold two buffers, chosen by segment parity; 0: one
     int keep0;                  // first kept segment (segments keep0.. have buffers of their own,
                                 // filled by the forward sweep); the segment count when none is kept

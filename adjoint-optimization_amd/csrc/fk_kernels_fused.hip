@@ -489,10 +489,15 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
         if ((tid & 63) == 0) sP[ly[sl] - G::H] = rs;
     }
     __syncthreads();
-    if (tid == 0) {
-        double s = 0.0;
-        for (int j = 0; j < TY; j++) s += sP[j];
-        gmem(S.partial)[(size_t)row * S.ntiles + t.tile] = s;
+    // one partial per 64 x PART_ROWS block of the tile (blocks below the grid are not written)
+    static_assert(TY % PART_ROWS == 0, "tiles hold whole partial blocks");
+    if (tid < TY / PART_ROWS) {
+        const int brow = t.y0 / PART_ROWS + tid;
+        if (brow * PART_ROWS < S.rows) {
+            double s = 0.0;
+            for (int j = 0; j < PART_ROWS; j++) s += sP[tid * PART_ROWS + j];
+            gmem(S.partial)[(size_t)row * S.ntiles + (size_t)brow * (S.pitch / FUSED_TX) + t.x0 / FUSED_TX] = s;
+        }
     }
 }
 

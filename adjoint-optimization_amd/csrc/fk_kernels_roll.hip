@@ -371,7 +371,7 @@ template <int P>
 struct RollAdj {
     Model32 m;
     float dt, E3, dEB;
-    int rows, n, c, y0, y1, tb, tyA, ntiles, row_i;
+    int rows, n, c, y0, y1, tb, ntiles, row_i;
     __amdgpu_buffer_rsrc_t zin, yA, yB, kA, kB, zout, code;
     double *partial;
     Rings<P, 5> X;        // rings 0..4: z, y(t_{i-1} + dt/3), Z1, Z2, Z3
@@ -556,11 +556,11 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         st3<P>(A.zout, row, A.n, A.c4, zo);
         const double rs = wave_sum64(zgh_term(m, act, cc, zo[0]));
         if ((c & 63) == 0) {
-            const int j = (row - A.y0) % A.tyA;
+            const int j = row % PART_ROWS;
             if (j == 0) A.tacc = 0.0;
             A.tacc += rs;
-            if (j == A.tyA - 1 || row == A.rows - 1)
-                gmem(A.partial)[(size_t)A.row_i * A.ntiles + (size_t)(row / A.tyA) * (P / 64) + (c >> 6)] = A.tacc;
+            if (j == PART_ROWS - 1 || row == A.rows - 1)
+                gmem(A.partial)[(size_t)A.row_i * A.ntiles + (size_t)(row / PART_ROWS) * (P / 64) + (c >> 6)] = A.tacc;
         }
     }
     if (t >= A.tb + 4)
@@ -610,7 +610,6 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
     A.code = buf_rsrc(S.code);
     A.partial = S.partial;
     A.ntiles = S.ntiles;
-    A.tyA = S.atile_rows;
     A.c = threadIdx.x;
     A.c4 = (uint32_t)A.c * 4u;
     A.c2 = (uint32_t)A.c * 2u;

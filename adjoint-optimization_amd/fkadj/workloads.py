@@ -109,6 +109,24 @@ def spiral_states(masks, seeds, chunk_ms: float = 5.0, settle_ms: float = 150.0,
     return states
 
 
+def holes_for(N: int) -> int:
+    """Disc-hole count at the C2 density (80 holes on 512^2)."""
+    return max(1, int(round(80 * N * N / 512 ** 2)))
+
+
+def prepare(sizes, seeds):
+    """(masks, states): disc holes at the C2 density and S1-S2 spiral states for instances of the
+    given square sizes (states prepared in shared launches per size)."""
+    masks = [holes_mask(N, holes_for(N), s) for N, s in zip(sizes, seeds)]
+    states = [None] * len(sizes)
+    for N in sorted(set(sizes)):
+        idx = [k for k, n in enumerate(sizes) if n == N]
+        st = spiral_states([masks[k] for k in idx], [seeds[k] for k in idx])
+        for k, y in zip(idx, st):
+            states[k] = y
+    return masks, states
+
+
 def phase_singularities(y: np.ndarray, mask: np.ndarray) -> int:
     """Spiral cores of a packed state: clusters of 2x2 plaquettes of active cells around which the
     phase atan2(v - 1/2, u - 1/2) winds by a full turn."""

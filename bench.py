@@ -347,21 +347,13 @@ def run_c5(args):
 
 
 def holes_for(N: int) -> int:
-    """Disc-hole count at the C2 density (80 holes on 512^2)."""
-    return max(1, int(round(80 * N * N / 512 ** 2)))
+    from fkadj import workloads as W
+    return W.holes_for(N)
 
 
 def prepare(sizes, seeds):
-    """S1-S2 spiral states for instances of the given sizes (prepared in shared launches per size)."""
     from fkadj import workloads as W
-    masks = [W.holes_mask(N, holes_for(N), s) for N, s in zip(sizes, seeds)]
-    states = [None] * len(sizes)
-    for N in sorted(set(sizes)):
-        idx = [k for k, n in enumerate(sizes) if n == N]
-        st = W.spiral_states([masks[k] for k in idx], [seeds[k] for k in idx])
-        for k, y in zip(idx, st):
-            states[k] = y
-    return masks, states
+    return W.prepare(sizes, seeds)
 
 
 def run_c3(args):
@@ -580,9 +572,8 @@ def main():
     states = W.spiral_states(masks, seeds)
     log(f"[bench] states ready in {time.perf_counter() - t0:.1f} s")
     A = [W.amplitudes(K_amp, s) for s in seeds]
-    from fkadj import autotune
-    _, adj_tile = autotune.lookup(N)   # the forward tile follows each launch's size (built-in rule)
-    fwd_tile = 0
+    # both tiles follow each launch's size (built-in rule; neither changes a result)
+    adj_tile = fwd_tile = 0
 
     def make_batch(keep_bytes):
         out = []

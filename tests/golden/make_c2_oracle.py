@@ -1,0 +1,65 @@
+"""Write tests/golden/c2_oracle.json: the FP64 oracle's gradient of one C2-size instance whose inputs
+are rebuilt from seeds on any machine (no GPU-prepared state), for the GPU test that pins the FP64
+GPU path at full C2 size (tests/test_gpu_configs.py).
+
+Instance: 512 x 512 grid with 80 seeded disc holes (radius 2-6 cells, isthmuses filled), h = 0.025 cm,
+dt = 0.02 ms, T = 200 ms (10 000 steps), 200 piecewise-constant amplitudes, objective
+mean(u(T)^2) + 1e-3 * pulse energy, checkpoints every 50 steps; initial state: a curved excitation front
+(u a logistic profile across a sinusoidal front line, v and w following it), the same builder as the
+small parity cases.  The oracle is the test infrastructure's FP64 C restatement (oracle/fk_oracle.c).
+
+    python tests/golden/make_c2_oracle.py      # about 4 minutes on 8 cores
+"""
+from __future__ import annotations
+
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(os.path.dirname(HERE))
+for p in (os.path.join(ROOT, "adjoint-optimization_amd"), os.path.join(ROOT, "oracle"), os.path.join(ROOT, "tests")):
+    sys.path.insert(0, p)
+
+N, HOLES, SEED, T, K = 512, 80, 11, 200.0, 200
+
+
+def inputs():
+    """(case, amplitudes, model, S): the instance, rebuilt from its seeds."""
+    import oracle as O
+    from cases import Case
+    from fkadj import geometry as G
+    from fkadj import workloads as W
+    mask = W.holes_mask(N, HOLES, SEED)
+    rng = np.random.default_rng(SEED + 1)
+    I, J = np.mgrid[0:N, 0:N]
+    front = 0.35 * N + 0.15 * N * np.sin(2 * np.pi * I / N + rng.uniform(0, 6))
+    with np.errstate(over="ignore"):
+        u = 1.0 / (1.0 + np.exp((J - front) / 1.5))
+    case = Case(mask, np.stack([u, 1.0 - 0.9 * u, 0.95 - 0.3 * u]), h=W.H_CM)
+    A = W.amplitudes(K, SEED)
+    return case, A, O.reference_model(h=W.H_CM), G.mean_u2_S(case.n_act)
+
+
+def main():
+    import oracle as O
+    from cases import oracle_grad_piecewise
+    from fkadj import workloads as W
+    case, A, model, S = inputs()
+    nt = int(round(T / W.DT_MS))
+    t0 = time.perf_counter()
+    g, Lc, L0, LE = oracle_grad_piecewise(case, W.DT_MS, nt, A, (1.0, 0.0, 0.0), 2 * W.LAMBDA, S, model)
+    sec = time.perf_counter() - t0
+    out = {"grid": N, "holes": HOLES, "seed": SEED, "T": T, "nt": nt, "n_amp": K, "ckpt": 50,
+           "n_active": case.n_act, "grad": [float(x) for x in g], "Lcost": float(Lc), "L0": float(L0),
+           "LE": float(LE), "oracle_seconds": sec, "oracle_threads": O.max_threads()}
+    with open(os.path.join(HERE, "c2_oracle.json"), "w") as f:
+        json.dump(out, f, indent=0)
+    print(f"c2_oracle.json: |grad| {np.linalg.norm(g):.6g}, Lcost {Lc:.12g}, {sec:.0f} s")
+
+
+if __name__ == "__main__":
+    main()

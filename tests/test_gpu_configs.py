@@ -1,0 +1,187 @@
+"""GPU tests on the benchmark configurations at their full sizes:
+
+* C1 -- 128^2, 20 holes, one S1-S2 spiral, T = 100 ms (5000 steps), 100 amplitudes: the plain FP64 GPU
+  path within 1e-9 of the FP64 oracle, the fused FP32 path within 1e-4;
+* C2 -- 512^2, 80 holes, T = 200 ms (10 000 steps), 200 amplitudes: the FP64 GPU path within 1e-9 of the
+  oracle's gradient stored by tests/golden/make_c2_oracle.py (inputs rebuilt from seeds), FP32 within 1e-4;
+* C3 -- 64 instances with grids drawn from {256, 384, 512, 768, 1024}^2, own holes and spirals,
+  T = 150 ms, 150 amplitudes: the captured plan's replay equals the batch call bit for bit, one member of
+  every grid size equals itself evaluated alone, the smallest member's FP32 gradient is within 1e-4 of
+  its FP64 gradient;
+* C4 -- a Poisson stream (200 req/s) of 24 requests with grids uniform over multiples of 64 in
+  [256, 1024] and T uniform in [50, 200] ms (1 ms amplitude pieces), admitted against a 200 GB budget:
+  every served result equals the request evaluated alone bit for bit.
+
+Reference path: fk_grad_Et_comp, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938.
+"""
+import importlib.util
+import json
+import os
+import time
+
+import numpy as np
+import pytest
+
+import fkadj
+import oracle as O
+from cases import Case, oracle_grad_piecewise, relerr
+from fkadj import autotune
+from fkadj import geometry as G
+from fkadj import workloads as W
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _print_case(name, **kw):
+    print(f"[case] {name} " + " ".join(f"{k}={v}" for k, v in kw.items()), flush=True)
+
+
+def _case_of(mask, y0):
+    ai = G.active_index_from_mask(mask)
+    planes = O.num2nan(y0, ai, *mask.shape)
+    return Case(mask, np.nan_to_num(planes), h=W.H_CM)
+
+
+@pytest.mark.timeout(300)
+def test_c1_full_size_fp64_and_fp32_against_oracle():
+    cfg = W.CONFIGS["C1"]
+    N, T, K = cfg["N"], cfg["T"], cfg["K"]
+    nt = int(round(T / W.DT_MS))
+    _print_case("c1", N=N, nt=nt, K=K)
+    m = W.holes_mask(N, cfg["holes"], 1)
+    y0 = W.spiral_states([m], [1])[0]
+    A = W.amplitudes(K, 1)
+    got = {}
+    for prec in (fkadj.F64, fkadj.F32):
+        inst = W.make_instance(m, T, K, precision=prec, ckpt=50)
+        inst.set_state(y0)
+        got[prec] = inst.grad(A)
+        inst.close()
+    case = _case_of(m, y0)
+    g, Lc, L0, LE = oracle_grad_piecewise(case, W.DT_MS, nt, A, (1.0, 0.0, 0.0), 2 * W.LAMBDA,
+                                          G.mean_u2_S(case.n_act), O.reference_model(h=W.H_CM))
+    e64, e32 = relerr(got[fkadj.F64][0], g), relerr(got[fkadj.F32][0], g)
+    print(f"[c1] FP64 vs oracle {e64:.2e}, FP32 vs oracle {e32:.2e}", flush=True)
+    assert e64 < 1e-9 and abs(got[fkadj.F64][1] / Lc - 1) < 1e-10
+    assert e32 < 1e-4 and abs(got[fkadj.F32][1] / Lc - 1) < 1e-4
+
+
+def _load_c2_fixture():
+    spec = importlib.util.spec_from_file_location("make_c2_oracle", os.path.join(HERE, "golden", "make_c2_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    with open(os.path.join(HERE, "golden", "c2_oracle.json")) as f:
+        return mod, json.load(f)
+
+
+@pytest.mark.timeout(300)
+def test_c2_full_size_fp64_against_stored_oracle_gradient():
+    mod, ref = _load_c2_fixture()
+    case, A, _, _ = mod.inputs()
+    assert case.n_act == ref["n_active"]
+    _print_case("c2_oracle", N=ref["grid"], nt=ref["nt"], K=ref["n_amp"])
+    got = {}
+    for prec in (fkadj.F64, fkadj.F32):
+        inst = fkadj.Instance(case.ai, ref["grid"], ref["grid"], dt=W.DT_MS, nt=ref["nt"], model=W.model(),
+                              pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=ref["n_amp"], alpha=2 * W.LAMBDA,
+                              gamma=(1.0, 0.0, 0.0), s_diag=2.0 / case.n_act, precision=prec, ckpt=ref["ckpt"])
+        inst.set_state(case.y0)
+        got[prec] = inst.grad(A)
+        inst.close()
+    g = np.array(ref["grad"])
+    e64, e32 = relerr(got[fkadj.F64][0], g), relerr(got[fkadj.F32][0], g)
+    print(f"[c2] FP64 vs oracle {e64:.2e}, FP32 vs oracle {e32:.2e}", flush=True)
+    assert e64 < 1e-9 and abs(got[fkadj.F64][1] / ref["Lcost"] - 1) < 1e-10
+    assert e32 < 1e-4 and abs(got[fkadj.F32][1] / ref["Lcost"] - 1) < 1e-4
+
+
+@pytest.mark.timeout(600)
+def test_c3_full_size_plan_batch_and_alone():
+    cfg = W.CONFIGS["C3"]
+    B, T, K = cfg["count"], cfg["T"], cfg["K"]
+    rng = np.random.default_rng(3)
+    sizes = [int(x) for x in rng.choice(cfg["sizes"], B)]
+    seeds = list(range(100, 100 + B))
+    _print_case("c3", count=B, sizes=sorted(set(sizes)), T=T)
+    t0 = time.perf_counter()
+    masks, states = W.prepare(sizes, seeds)
+    print(f"[c3] states ready in {time.perf_counter() - t0:.1f} s", flush=True)
+    A = [W.amplitudes(K, s) for s in seeds]
+    insts = []
+    for m, y in zip(masks, states):
+        inst = W.make_instance(m, T, K, precision=fkadj.F32, ckpt=50, adj_tile=autotune.lookup(m.shape[0])[1])
+        inst.set_state(y)
+        insts.append(inst)
+    batched = fkadj.batch_grad(insts, A)
+    with fkadj.Plan(insts) as plan:
+        replay = plan.grad(A)
+    for a, b in zip(batched, replay):
+        assert np.array_equal(a[0], b[0]) and a[1] == b[1]
+    for N in sorted(set(sizes)):
+        k = sizes.index(N)
+        alone = insts[k].grad(A[k])
+        assert np.array_equal(alone[0], batched[k][0]) and alone[1] == batched[k][1], N
+    k0 = int(np.argmin(sizes))
+    i64 = W.make_instance(masks[k0], T, K, precision=fkadj.F64, ckpt=50)
+    i64.set_state(states[k0])
+    g64, Lc64, _, _ = i64.grad(A[k0])
+    i64.close()
+    e = relerr(batched[k0][0], g64)
+    print(f"[c3] smallest member ({sizes[k0]}^2) FP32 vs FP64: {e:.2e}", flush=True)
+    assert e < 1e-4 and abs(batched[k0][1] / Lc64 - 1) < 1e-4
+    for inst in insts:
+        inst.close()
+
+
+def _front_state(mask, seed):
+    """A curved excitation front (u logistic across a sinusoidal line, v and w following it), packed."""
+    H, Wd = mask.shape
+    rng = np.random.default_rng(seed)
+    I, J = np.mgrid[0:H, 0:Wd]
+    front = 0.35 * Wd + 0.15 * Wd * np.sin(2 * np.pi * I / H + rng.uniform(0, 6))
+    with np.errstate(over="ignore"):
+        u = 1.0 / (1.0 + np.exp((J - front) / 1.5))
+    return np.concatenate([u[mask], (1.0 - 0.9 * u)[mask], (0.95 - 0.3 * u)[mask]])
+
+
+@pytest.mark.timeout(600)
+def test_c4_poisson_stream_at_full_size_equals_alone():
+    rng = np.random.default_rng(4)
+    R, rate, budget = 24, 200.0, 200 * 10 ** 9
+    sizes = [int(x) for x in rng.choice(np.arange(256, 1025, 64), R)]
+    Ts = [int(x) for x in rng.integers(50, 201, R)]
+    _print_case("c4", requests=R, sizes=sizes, T=Ts)
+    insts, Es = [], []
+    for k, (N, T) in enumerate(zip(sizes, Ts)):
+        m = W.holes_mask(N, W.holes_for(N), 300 + k)
+        inst = fkadj.Instance(G.active_index_from_mask(m), N, N, dt=W.DT_MS, nt=int(round(T / W.DT_MS)),
+                              model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=T, alpha=2 * W.LAMBDA,
+                              gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()), precision=fkadj.F32, ckpt=50,
+                              adj_tile=autotune.lookup(N)[1], lazy_workspace=True)
+        inst.set_state(_front_state(m, 300 + k))
+        insts.append(inst)
+        Es.append(W.amplitudes(T, 1000 + k))
+    arrivals = np.cumsum(rng.exponential(1.0 / rate, R))
+    served = {}
+    with fkadj.Server(budget_bytes=budget, max_slots=64) as srv:
+        ids, nxt, start = [None] * R, 0, time.perf_counter()
+        while len(served) < R:
+            now = time.perf_counter() - start
+            while nxt < R and arrivals[nxt] <= now:
+                ids[nxt] = srv.submit(insts[nxt], Es[nxt])
+                nxt += 1
+            if srv.pending() == 0:
+                time.sleep(0.0005)
+                continue
+            if srv.run(1) > 0:
+                for r in range(nxt):
+                    if r not in served:
+                        out = srv.poll(ids[r])
+                        if out is not None:
+                            served[r] = out
+    for r in range(R):
+        alone = insts[r].grad(Es[r])
+        assert np.array_equal(alone[0], served[r][0]) and alone[1] == served[r][1], (r, sizes[r], Ts[r])
+    for inst in insts:
+        inst.close()

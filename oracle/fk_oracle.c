@@ -21,6 +21,10 @@
  * kept as checkpoints every `ckpt` steps and recomputed per segment; the recomputation is the
  * same code in the same order, so the result equals a store-all run bit for bit.
  *
+ * Precision probe (not part of the reference's algorithm): with round_state != 0, orc_grad rounds the
+ * state to FP32 after every forward and recompute step, so that a test can measure what FP32 storage of
+ * the trajectory alone, with FP64 arithmetic everywhere else, does to the gradient.
+ *
  * Grid convention: cell k = i*ncols + j (j contiguous); nx = nrows, ny = ncols (G:25-28).
  * Build: gcc -O2 -fopenmp -ffp-contract=off -fPIC -shared (no fast-math: NaN semantics matter).
  */
@@ -392,10 +396,16 @@ static double herm4_part(double x0, double x1, double m0, double m1)
  * CSC matrix S (n_num x n_num) with active_index mapping, gamma[3].
  * Outputs: zgh[nt+1], L0 (= sum_v gamma_v y_v^T S y_v at T), yT (final state, 3 planes, may be NULL).
  * ckpt: forward checkpoint interval (>= 1). */
+/* State values to the nearest FP32 (NaN holes stay NaN): the precision probe of orc_grad. */
+static void round_to_f32(double *p, size_t n)
+{
+    for (size_t k = 0; k < n; k++) p[k] = (double)(float)p[k];
+}
+
 int orc_grad(const orc_model *m, int nx, int ny, double dt, int nt, const double *Ef, const double *Ea,
              const double *gh, const double *y0, const int32_t *active_index, int n_num,
              const int64_t *S_colptr, const int64_t *S_rowind, const double *S_val, const double *gamma,
-             int ckpt, double *zgh, double *L0_out, double *yT)
+             int ckpt, double *zgh, double *L0_out, double *yT, int round_state)
 {
     size_t n = (size_t)nx * ny, n3 = 3 * n;
     if (ckpt < 1) ckpt = 1;
@@ -416,9 +426,11 @@ int orc_grad(const orc_model *m, int nx, int ny, double dt, int nt, const double
 
     /* forward with checkpoints */
     memcpy(y, y0, sizeof(double) * n3);
+    if (round_state) round_to_f32(y, n3);
     memcpy(ck, y, sizeof(double) * n3);
     for (int i = 0; i < nt; i++) {
         rk4_step(m, nx, ny, dt, gh, Ef + 3 * i, y, &ws);
+        if (round_state) round_to_f32(y, n3);
         if ((i + 1) % ckpt == 0) memcpy(ck + n3 * ((i + 1) / ckpt), y, sizeof(double) * n3);
     }
     if (yT) memcpy(yT, y, sizeof(double) * n3);
@@ -452,6 +464,7 @@ int orc_grad(const orc_model *m, int nx, int ny, double dt, int nt, const double
         for (int i = a; i < b; i++) {
             memcpy(seg + n3 * (i - a + 1), seg + n3 * (i - a), sizeof(double) * n3);
             rk4_step(m, nx, ny, dt, gh, Ef + 3 * i, seg + n3 * (i - a + 1), &ws);
+            if (round_state) round_to_f32(seg + n3 * (i - a + 1), n3);
         }
         for (int i = b; i > a; i--) {
             const double *yA = seg + n3 * (i - 1 - a);   /* y(t_{i-1}) */
@@ -559,7 +572,7 @@ int orc_grad_ref(double dt, const double *Et, int nt, const double *y_ini, const
     rc = orc_knot_tables(dt, nt, Et, Ef, Ea);
     if (rc) goto out;
     rc = orc_grad(model, nrows, ncols, dt, nt, Ef, Ea, gh, y, active_index, n_num, S_colptr, S_rowind, S_val,
-                  gamma, ckpt, zgh, L0, NULL);
+                  gamma, ckpt, zgh, L0, NULL, 0);
     if (rc) goto out;
     double e2 = 0.0;
     for (int i = 0; i <= nt; i++) e2 += Et[i] * Et[i];

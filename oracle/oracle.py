@@ -61,7 +61,7 @@ def lib():
         L.orc_knot_tables.argtypes = [C.c_double, C.c_int, dp, dp, dp]
         L.orc_forward.argtypes = [mp, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp, dp]
         L.orc_grad.argtypes = [mp, C.c_int, C.c_int, C.c_double, C.c_int, dp, dp, dp, dp, ip, C.c_int,
-                               lp, lp, dp, dp, C.c_int, dp, dp, dp]
+                               lp, lp, dp, dp, C.c_int, dp, dp, dp, C.c_int]
         L.orc_num2nan.argtypes = [dp, ip, C.c_int, C.c_int, dp]
         L.orc_grad_ref.argtypes = [C.c_double, dp, C.c_int, dp, ip, C.c_int, C.c_int, C.c_int, dp, dp,
                                    lp, lp, dp, C.c_double, mp, C.c_int, dp, dp, dp, dp]
@@ -161,8 +161,12 @@ def forward(m: Model, y_planes: np.ndarray, gh: np.ndarray, dt: float, Ef: np.nd
     return y
 
 
-def grad(m: Model, y_planes, gh, dt, Ef, Ea, active_index, S_csc, gamma, ckpt=50):
-    """Generic gradient evaluation; returns (zgh[nt+1], L0, yT planes)."""
+def grad(m: Model, y_planes, gh, dt, Ef, Ea, active_index, S_csc, gamma, ckpt=50, round_state=False):
+    """Generic gradient evaluation; returns (zgh[nt+1], L0, yT planes).
+
+    round_state=True is a precision probe, not the reference's algorithm: the trajectory is rounded to
+    FP32 after every forward and recompute step (FP64 arithmetic otherwise), which measures how much of
+    an FP32 gradient's error the storage of the state alone accounts for."""
     y = _f64(y_planes); gh = _f64(gh); Ef = _f64(Ef); Ea = _f64(Ea)
     ai = np.ascontiguousarray(active_index, dtype=np.int32)
     colptr, rowind, val = (np.ascontiguousarray(S_csc[0], dtype=np.int64),
@@ -176,7 +180,7 @@ def grad(m: Model, y_planes, gh, dt, Ef, Ea, active_index, S_csc, gamma, ckpt=50
     rc = lib().orc_grad(C.byref(m), y.shape[1], y.shape[2], dt, nt, _d(Ef), _d(Ea), _d(gh), _d(y),
                         ai.ctypes.data_as(C.POINTER(C.c_int32)), n_num,
                         colptr.ctypes.data_as(C.POINTER(C.c_int64)), rowind.ctypes.data_as(C.POINTER(C.c_int64)),
-                        _d(val), _d(gamma), int(ckpt), _d(zgh), _d(L0), _d(yT))
+                        _d(val), _d(gamma), int(ckpt), _d(zgh), _d(L0), _d(yT), int(bool(round_state)))
     if rc:
         raise RuntimeError(f"orc_grad failed ({rc})")
     return zgh, float(L0[0]), yT

@@ -69,6 +69,25 @@ def oracle_grad_knots(case, dt, Et, gamma, alpha, S, model=None, ckpt=50):
                       model=model, ckpt=ckpt)
 
 
+def oracle_grad_knots_state_rounded(dt, Et, y_ini, active_index, nrows, ncols, gh_num, gamma, S, alpha, model=None,
+                                    ckpt=50, round_state=True):
+    """The knot-mode oracle gradient (gradE, Lcost) assembled from orc_grad; with round_state the
+    trajectory is rounded to FP32 after every step and FP64 arithmetic is kept otherwise, which gives
+    the error that FP32 storage of the state alone puts into gradE."""
+    model = model or O.reference_model()
+    nt = len(Et) - 1
+    planes = O.num2nan(y_ini, active_index, nrows, ncols)
+    ai = np.asarray(active_index)
+    gh = np.full(nrows * ncols, np.nan)
+    gh[ai > 0] = model.Du * 2 / model.h * np.asarray(gh_num, float)[ai[ai > 0] - 1]
+    Ef, Ea = O.knot_tables(dt, Et)
+    zgh, L0, _ = O.grad(model, planes, gh.reshape(nrows, ncols), dt, Ef, Ea, active_index, G.csc_arrays(S), gamma,
+                        ckpt=ckpt, round_state=round_state)
+    Et = np.asarray(Et, float)
+    LE = dt * float(np.sum(Et * Et))
+    return alpha * Et - zgh, 0.5 * L0 + 0.5 * alpha * LE
+
+
 def oracle_grad_piecewise(case, dt, nt, A, gamma, alpha, S, model):
     """Oracle gradient in piecewise mode: E constant over each step; dJ/dA_k by trapezoid of
     alpha*E - z.gh over the steps of piece k.  Returns (gradA, Lcost, L0, LE)."""

@@ -8,7 +8,7 @@ mean(u(T)^2) + 1e-3 * pulse energy, checkpoints every 50 steps; initial state: a
 (u a logistic profile across a sinusoidal front line, v and w following it), the same builder as the
 small parity cases.  The oracle is the test infrastructure's FP64 C restatement (oracle/fk_oracle.c).
 
-    python tests/golden/make_c2_oracle.py      # about 4 minutes on 8 cores
+    python tests/golden/make_c2_oracle.py      # about 15 minutes on 8 cores
 """
 from __future__ import annotations
 

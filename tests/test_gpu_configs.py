@@ -10,7 +10,13 @@
   its FP64 gradient;
 * C4 -- a Poisson stream (200 req/s) of 24 requests with grids uniform over multiples of 64 in
   [256, 1024] and T uniform in [50, 200] ms (1 ms amplitude pieces), admitted against a 200 GB budget:
-  every served result equals the request evaluated alone bit for bit.
+  every served result equals the request evaluated alone bit for bit;
+* C5 -- 2048^2, 400 holes, four interacting spirals, T = 500 ms (25 000 steps), 500 amplitudes: the FP32
+  gradient within 5e-3 of the FP64 path and J within 1e-4;
+* the C5 horizon on a 256^2 spiral (T = 500 ms, 500 amplitudes): the FP64 GPU path within 1e-9 of the
+  oracle's gradient stored by tests/golden/make_horizon_oracle.py over all 25 000 steps, and the FP32
+  path within 8 times the error that FP32 storage of the state alone causes (the same oracle rounding
+  its trajectory to FP32 after every step).
 
 Reference path: fk_grad_Et_comp, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938.
 """
@@ -185,3 +191,61 @@ def test_c4_poisson_stream_at_full_size_equals_alone():
         assert np.array_equal(alone[0], served[r][0]) and alone[1] == served[r][1], (r, sizes[r], Ts[r])
     for inst in insts:
         inst.close()
+
+
+@pytest.mark.timeout(600)
+def test_c5_full_horizon_fp32_vs_fp64():
+    """Over 500 ms the four-spiral state amplifies per-step rounding: the FP32 gradient's error against
+    the FP64 path is 1.5e-3 (under 1e-4 over the first 100 ms), while perturbing the initial state by
+    FP32 rounding moves the FP64 gradient by 2e-7 only; the rounding injected at every step is what
+    grows (on the 256^2 horizon case below the FP32-storage floor is measured directly)."""
+    cfg = W.CONFIGS["C5"]
+    N, T, K = cfg["N"], cfg["T"], cfg["K"]
+    _print_case("c5_full_horizon", N=N, T=T, K=K)
+    m = W.holes_mask(N, cfg["holes"], 5)
+    y0 = W.spiral_states([m], [5], bands=2)[0]
+    A = W.amplitudes(K, 5)
+    out = {}
+    for prec in (fkadj.F32, fkadj.F64):
+        t0 = time.perf_counter()
+        inst = W.make_instance(m, T, K, precision=prec, ckpt=cfg["ckpt"])
+        inst.set_state(y0)
+        out[prec] = inst.grad(A)
+        inst.close()
+        print(f"[c5] precision {prec}: {time.perf_counter() - t0:.1f} s", flush=True)
+    e = relerr(out[fkadj.F32][0], out[fkadj.F64][0])
+    eJ = abs(out[fkadj.F32][1] / out[fkadj.F64][1] - 1)
+    print(f"[c5] T=500 ms FP32 vs FP64: grad {e:.3e}, J {eJ:.3e}", flush=True)
+    assert e < 5e-3 and eJ < 1e-4
+
+
+def _load_horizon_fixture():
+    spec = importlib.util.spec_from_file_location("make_horizon_oracle",
+                                                  os.path.join(HERE, "golden", "make_horizon_oracle.py"))
+    mod = importlib.util.module_from_spec(spec)
+    spec.loader.exec_module(mod)
+    with open(os.path.join(HERE, "golden", "horizon_oracle.json")) as f:
+        return mod, json.load(f)
+
+
+@pytest.mark.timeout(300)
+def test_horizon_fp64_against_oracle_and_fp32_against_storage_floor():
+    mod, ref = _load_horizon_fixture()
+    case, A, _, _ = mod.inputs()
+    assert case.n_act == ref["n_active"]
+    _print_case("horizon_oracle", N=ref["grid"], nt=ref["nt"], K=ref["n_amp"])
+    got = {}
+    for prec in (fkadj.F64, fkadj.F32):
+        inst = fkadj.Instance(case.ai, ref["grid"], ref["grid"], dt=W.DT_MS, nt=ref["nt"], model=W.model(),
+                              pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=ref["n_amp"], alpha=2 * W.LAMBDA,
+                              gamma=(1.0, 0.0, 0.0), s_diag=2.0 / case.n_act, precision=prec, ckpt=ref["ckpt"])
+        inst.set_state(case.y0)
+        got[prec] = inst.grad(A)
+        inst.close()
+    g = np.array(ref["grad"])
+    floor = relerr(ref["grad_state_fp32"], g)
+    e64, e32 = relerr(got[fkadj.F64][0], g), relerr(got[fkadj.F32][0], g)
+    print(f"[horizon] FP64 vs oracle {e64:.2e}; FP32 vs oracle {e32:.2e} = {e32 / floor:.1f} x the FP32-state "
+          f"floor {floor:.2e}", flush=True)
+    assert e64 < 1e-9 and abs(got[fkadj.F64][1] / ref["Lcost"] - 1) < 1e-10
+    assert e32 < 8 * floor and abs(got[fkadj.F32][1] / ref["Lcost"] - 1) < 1e-4

@@ -4,8 +4,9 @@ Tolerances:
   * FP64 GPU path vs oracle: relative 1e-9 (different tanh/FMA rounding, amplified by the dynamics);
     vs the reference's golden gradE at 256^2 / 3000 steps: relative 1e-9.
   * FP32 path vs FP64: relative 1e-4 on J and on the gradient on the short parity cases and on a
-    C2-type workload (the target of the design); the reference's own 3000-step golden case is more
-    sensitive (see test_golden_256_fp32_error_vs_fp64).
+    C2-type workload (the target of the design); on the reference's own 3000-step golden case the
+    gradient error is bounded by 8 times the error of FP32 state storage alone
+    (test_golden_256_fp32_error_vs_fp64).
   * batched vs alone, checkpoint interval, tile height: bit-identical (same kernels, fixed order).
 """
 import numpy as np
@@ -15,7 +16,8 @@ import fkadj
 from fkadj import _lib as L
 from fkadj import geometry as G
 
-from cases import crop_case, disc_case, golden, oracle_grad_knots, oracle_grad_piecewise, relerr
+from cases import (crop_case, disc_case, golden, oracle_grad_knots, oracle_grad_knots_state_rounded,
+                   oracle_grad_piecewise, relerr)
 
 pytestmark = pytest.mark.gpu
 
@@ -382,13 +384,16 @@ def test_golden_256_gradient_descent_iterations_1_to_3():
 
 
 @pytest.mark.slow
+@pytest.mark.timeout(300)
 def test_golden_256_fp32_error_vs_fp64():
     """FP32 fused path on the reference's own case (256^2, dt = 0.1 ms, 3000 steps, mostly one-cell
-    holes).  J stays within 1e-5 of the FP64 golden value; the gradient is far more sensitive here:
-    FP32 rounding inside the right-hand side at the steep fronts grows to 4e-5 in the final state
-    (a compensated state update only brings that to 2e-5), and z.gh cancels across one-cell holes,
-    so the measured gradient error is 8.7e-3.  The 1e-4 gradient bound is held on the C2-type
-    workload (test_fp32_c2_type_error_bound)."""
+    holes).  J stays within 1e-5 of the FP64 golden value.  The gradient cannot meet 1e-4 here with any
+    FP32 state: the oracle with FP64 arithmetic throughout, rounding only the stored state to FP32
+    after each step, is already 1.5e-3 from the golden gradE (perturbing the initial state alone by
+    FP32 rounding moves it by under 2e-4, so it is the rounding injected at every step at the steep
+    fronts, with z.gh cancelling across one-cell holes, that the case amplifies).  The FP32 path's
+    own error (7.8e-3) is held to at most 8 times that storage floor; the 1e-4 gradient bound is held
+    on the C2-type workloads."""
     _print_case("golden256_it50_fp32")
     d, meta = golden()
     mask = G.mask_from_active_index(d["active_index"], 256, 256)
@@ -398,8 +403,14 @@ def test_golden_256_fp32_error_vs_fp64():
         inst.set_state(d["y0"])
         g, Lc, _, _ = inst.grad(d["Et_50"])
     e = relerr(g, d["gradE_50"])
-    print(f"fp32 golden gradE rel err {e:.3e}, Lcost rel err {abs(Lc / meta['iterations']['50']['Lcost'] - 1):.3e}")
-    assert e < 3e-2
+    print(f"fp32 golden gradE rel err {e:.3e}, Lcost rel err {abs(Lc / meta['iterations']['50']['Lcost'] - 1):.3e}",
+          flush=True)
+    g_floor, _ = oracle_grad_knots_state_rounded(meta["dt"], d["Et_50"], d["y0"], d["active_index"], 256, 256,
+                                                 d["ghost_lr"].astype(float), meta["gamma"], S, meta["alpha"])
+    floor = relerr(g_floor, d["gradE_50"])
+    print(f"FP32-state floor {floor:.3e}; FP32 path at {e / floor:.1f} x floor", flush=True)
+    assert 1e-3 < floor < 3e-3
+    assert e < 8 * floor
     assert abs(Lc - meta["iterations"]["50"]["Lcost"]) < 1e-5 * meta["iterations"]["50"]["Lcost"]
 
 
@@ -573,9 +584,8 @@ def test_kept_segments_change_nothing():
 @pytest.mark.slow
 def test_fp32_c5_grid_short_horizon_error_bound():
     """The C5 grid at its full size (2048^2, 400 disc holes, four interacting spirals) over the first
-    100 ms (5000 steps, 100 amplitudes): FP32 within 1e-4 of FP64.  (Over the full 500 ms the
-    chaotic four-spiral state amplifies FP32 rounding to 1.5e-3; that horizon is reported by the
-    benchmark, not asserted.)"""
+    100 ms (5000 steps, 100 amplitudes): FP32 within 1e-4 of FP64.  (The full 500 ms horizon, where the
+    four-spiral state amplifies per-step rounding to 1.5e-3, is asserted in test_gpu_configs.py.)"""
     _print_case("c5_grid_fp32_vs_fp64", grid=2048, T=100)
     from fkadj import workloads as W
     cfg = W.CONFIGS["C5"]

@@ -194,3 +194,32 @@ def test_gradient_matches_finite_differences():
     e1, e2 = worst_mismatch(0.02, 40), worst_mismatch(0.01, 80)
     print(f"oracle adjoint vs central differences: max rel {e1:.2e} (dt 0.02), {e2:.2e} (dt 0.01)")
     assert e1 < 1e-3 and 3.5 < e1 / e2 < 4.5, (e1, e2)
+
+
+def test_state_rounding_probe():
+    """The precision probe of orc_grad (round_state): the final state it returns is exactly FP32
+    representable, the gradient moves by a rounding-sized amount (not zero, not order one) and the
+    knot-mode helper built on it agrees with grad_ref when the probe is off."""
+    from cases import disc_case, oracle_grad_knots, oracle_grad_knots_state_rounded
+    case = disc_case(24, 40, 2, 5)
+    om = O.reference_model(h=0.025)
+    S = G.S_matrix(case.mask)
+    nt, dt = 200, 0.02
+    Ef = np.repeat(np.linspace(2.0, 6.0, nt), 3)
+    gh = case.gh_plane(om)
+    z0, L0, yT = O.grad(om, case.planes(), gh, dt, Ef, Ef, case.ai, G.csc_arrays(S), (1, 0.2, 0.2), ckpt=20)
+    z1, L1, yR = O.grad(om, case.planes(), gh, dt, Ef, Ef, case.ai, G.csc_arrays(S), (1, 0.2, 0.2), ckpt=20,
+                        round_state=True)
+    act = case.mask
+    assert np.array_equal(yR[:, act], yR[:, act].astype(np.float32).astype(np.float64))
+    assert not np.array_equal(yT[:, act], yR[:, act])
+    e = np.linalg.norm(z1 - z0) / np.linalg.norm(z0)
+    assert 1e-9 < e < 1e-3, e
+    Et = 3.0 + np.sin(np.arange(nt + 1) * 0.05)
+    args = (dt, Et, case.y0, case.ai, case.nrows, case.ncols, case.gh, (1, 0.2, 0.2), S, 2e-3)
+    g0, Lc0, _, _ = oracle_grad_knots(case, dt, Et, (1, 0.2, 0.2), 2e-3, S, model=om)
+    gf, Lcf = oracle_grad_knots_state_rounded(*args, model=om, round_state=False)
+    assert np.linalg.norm(gf - g0) / np.linalg.norm(g0) < 1e-13 and abs(Lcf / Lc0 - 1) < 1e-13
+    g, Lc = oracle_grad_knots_state_rounded(*args, model=om)
+    assert 0 < np.linalg.norm(g - g0) / np.linalg.norm(g0) < 1e-3
+    assert abs(Lc / Lc0 - 1) < 1e-5

@@ -212,11 +212,16 @@ struct FusedGroupDev {
     // a group of a split batch runs on a batch lane stream (from a host thread of its own), forked
     // from and joined to the caller's stream through ev_lane
     hipEvent_t ev_lane = nullptr;
-    // large groups of one pitch run the row-rolling kernels (one workgroup per strip of roll_hs rows)
-    // for their forward-kind and adjoint steps: the same results as the tiled kernels
-    bool roll = false;
-    int roll_pitch = 0, roll_hs = 16;
-    LaunchMap rollmap;
+    // the members of each pitch that the row-rolling kernels support, when the group holds 2M cells
+    // or more of that pitch, run them for their forward-kind and adjoint steps (one launch per such
+    // pitch, one workgroup per strip of roll_hs rows); the other members run the tiled kernels
+    // (fwd_rest, adj_rest).  The same results either way.  f(y_nt) (MODE_RHS) always takes the tiled
+    // maps of all members.
+    static constexpr int kMaxRoll = 5;
+    int nroll = 0, roll_hs = 16;
+    int roll_pitch[kMaxRoll] = {};
+    LaunchMap rollmap[kMaxRoll];
+    LaunchMap fwd_rest[NUM_TILE_CFG], adj_rest[NUM_TILE_CFG];
 };
 
 namespace {
@@ -445,42 +450,47 @@ struct EventPool {
     }
 };
 
-constexpr int kSampleEvery = 8;   // profile one launch in 8
+// Launch timing: a window of kWindow consecutive launches of one kind (forward-kind or adjoint) on one
+// stream, opened once every kSampleEvery launches of that kind, is bracketed by two events; the mean
+// launch duration is the windows' elapsed time over their launches.  A window spans back-to-back
+// launches only, so the gap between the opening event and the first kernel is shared by eight
+// launches instead of charged to one; a launch of the other kind or a phase boundary (cut) drops
+// the open window.
+constexpr int kWindow = 8;
+constexpr int kSampleEvery = 64;
 
 struct Sampler {
     bool on;
     EventPool pool;
     std::vector<std::pair<hipEvent_t, hipEvent_t>> fwd, adj;
     int64_t nf = 0, na = 0;
+    hipEvent_t open = nullptr;   // the open window's first event
+    int open_kind = 0, open_n = 0;
     explicit Sampler(bool o) : on(o) {}
+    void cut() { open = nullptr; }
     template <typename F>
-    void fwd_launch(hipStream_t s, F f)
+    void launch(int kind, hipStream_t s, F f)
     {
-        if (on && (nf % kSampleEvery) == 0) {
-            hipEvent_t a = pool.get(), b = pool.get();
-            (void)hipEventRecord(a, s);
-            f();
-            (void)hipEventRecord(b, s);
-            fwd.push_back({a, b});
-        } else {
-            f();
+        int64_t &cnt = kind ? na : nf;
+        if (open && open_kind != kind) open = nullptr;
+        if (on && !open && cnt % kSampleEvery == 0) {
+            open = pool.get();
+            if (open && hipEventRecord(open, s) != hipSuccess) open = nullptr;
+            open_kind = kind;
+            open_n = 0;
         }
-        nf++;
+        f();
+        cnt++;
+        if (open && ++open_n == kWindow) {
+            hipEvent_t b = pool.get();
+            if (b && hipEventRecord(b, s) == hipSuccess) (kind ? adj : fwd).push_back({open, b});
+            open = nullptr;
+        }
     }
     template <typename F>
-    void adj_launch(hipStream_t s, F f)
-    {
-        if (on && (na % kSampleEvery) == 0) {
-            hipEvent_t a = pool.get(), b = pool.get();
-            (void)hipEventRecord(a, s);
-            f();
-            (void)hipEventRecord(b, s);
-            adj.push_back({a, b});
-        } else {
-            f();
-        }
-        na++;
-    }
+    void fwd_launch(hipStream_t s, F f) { launch(0, s, f); }
+    template <typename F>
+    void adj_launch(hipStream_t s, F f) { launch(1, s, f); }
     static double mean(const std::vector<std::pair<hipEvent_t, hipEvent_t>> &v)
     {
         if (v.empty()) return 0.0;
@@ -490,7 +500,7 @@ struct Sampler {
             (void)hipEventElapsedTime(&ms, p.first, p.second);
             s += ms;
         }
-        return s / v.size();
+        return s / (v.size() * (double)kWindow);
     }
 };
 
@@ -653,10 +663,12 @@ int run_plain(EvalCtx *C, Sampler &smp)
         smp.fwd_launch(s, [&] { plain_forward_step(pc, yin, yout, i); });
     }
     HIPCHK(hipGetLastError());
+    smp.cut();
     terminal<T>(C, s);
     const int nseg = (nt + K - 1) / K;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt);
+        smp.cut();
         for (int j = 0; j + 1 < b - a; j++) {
             T *yin = as<T>(j == 0 ? I->state(C->d_ck, sg) : I->state(C->d_seg, j));
             T *yout = as<T>(I->state(C->d_seg, j + 1));
@@ -737,22 +749,37 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
         g->depth2 = g->depth2 && I->fwd_cfg == 0;
     }
-    // row-rolling kernels: one pitch they support, strips a multiple of every adjoint tile height
+    // row-rolling kernels per pitch they support: every such pitch (mode 2) or those with 2M cells
+    // or more in the group (mode 1); strips of hs rows (a multiple of the partial block height)
     const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
-    bool roll = rmode == 2 || (rmode == 1 && cells >= kRollCells);
-    for (size_t k = 0; k < members.size() && roll; k++) {
-        const fkadj_instance *I = members[k]->I;
-        roll = I->pitch == I0->pitch && roll_pitch_supported(I->pitch);
-    }
-    g->roll = roll;
-    g->roll_pitch = I0->pitch;
+    std::map<int, int64_t> pitch_cells;
+    for (auto *C : members) pitch_cells[C->I->pitch] += (int64_t)C->I->rows * C->I->pitch;
+    g->nroll = 0;
     g->roll_hs = hs;
-    launch_map_clear(g->rollmap);
-    if (roll) {
-        g->depth2 = false;
-        for (size_t k = 0; k < members.size(); k++)
-            launch_map_add(g->rollmap, (int)k, (members[k]->I->rows + hs - 1) / hs);
+    for (auto &kv : pitch_cells)
+        if (roll_pitch_supported(kv.first) && (rmode == 2 || (rmode == 1 && kv.second >= kRollCells)) &&
+            g->nroll < FusedGroupDev::kMaxRoll) {
+            g->roll_pitch[g->nroll] = kv.first;
+            launch_map_clear(g->rollmap[g->nroll]);
+            g->nroll++;
+        }
+    for (int c = 0; c < NUM_TILE_CFG; c++) {
+        launch_map_clear(g->fwd_rest[c]);
+        launch_map_clear(g->adj_rest[c]);
     }
+    for (size_t k = 0; k < members.size(); k++) {
+        const fkadj_instance *I = members[k]->I;
+        int r = 0;
+        while (r < g->nroll && g->roll_pitch[r] != I->pitch) r++;
+        if (r < g->nroll) {
+            launch_map_add(g->rollmap[r], (int)k, (I->rows + hs - 1) / hs);
+        } else {
+            const int fc = fwd_cfg_of(I, cells), ac = adj_cfg_of(I, cells);
+            launch_map_add(g->fwd_rest[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
+            launch_map_add(g->adj_rest[ac], (int)k, tiles_of(I->rows, I->pitch, ac));
+        }
+    }
+    if (g->nroll) g->depth2 = false;
 }
 
 int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
@@ -809,10 +836,15 @@ void groups_free(std::vector<FusedGroupDev *> &gs)
 // The members of a batch call as groups, each run from its own host thread on its own stream: a
 // batch of 2M cells or more (its backward pass runs on one stream) is cut into up to max_groups
 // contiguous runs of members of about equal cells; a smaller one stays one group.
-int groups_build(const std::vector<EvalCtx *> &members, hipStream_t s, int max_groups,
+int groups_build(const std::vector<EvalCtx *> &members_in, hipStream_t s, int max_groups,
                  std::vector<FusedGroupDev *> &out)
 {
     out.clear();
+    // members of one pitch next to each other (stable), so that the groups hold few pitches with
+    // many cells each, which the row-rolling kernels take; results do not depend on the grouping
+    std::vector<EvalCtx *> members(members_in);
+    std::stable_sort(members.begin(), members.end(),
+                     [](const EvalCtx *a, const EvalCtx *b) { return a->I->pitch < b->I->pitch; });
     const int n = (int)members.size();
     std::vector<int64_t> cells(n);
     int64_t total = 0;
@@ -842,8 +874,9 @@ Cursors uniform_cursors(int count, int mode, int step, int seg_a)
 void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
-    if (g->roll && mode != MODE_RHS) {
-        roll_launch(false, g->roll_pitch, g->d_slots, g->rollmap, c, g->roll_hs, s);
+    if (g->nroll && mode != MODE_RHS) {
+        for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd_rest[k], c, s);
+        for (int r = 0; r < g->nroll; r++) roll_launch(false, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs, s);
         return;
     }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
@@ -858,8 +891,9 @@ void group_forward2(const FusedGroupDev *g, int mode, int step, int seg_a, hipSt
 void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
 {
     const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
-    if (g->roll) {
-        roll_launch(true, g->roll_pitch, g->d_slots, g->rollmap, c, g->roll_hs, s);
+    if (g->nroll) {
+        for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj_rest[k], c, s);
+        for (int r = 0; r < g->nroll; r++) roll_launch(true, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs, s);
         return;
     }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj[k], c, s);
@@ -885,6 +919,7 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
             smp.fwd_launch(s, [&] { group_forward(g, MODE_KEEP, i, i / K * K, s); });
     }
     HIPCHK(hipGetLastError());
+    smp.cut();
     for (auto *C : g->members) terminal<float>(C, s);
     group_forward(g, MODE_RHS, nt - 1, 0, s);   // f(y_nt) for the last adjoint step
     // Backward pass, two streams: the recompute of segment s-1 (this stream) runs while the adjoint
@@ -905,6 +940,7 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt), p = sg & 1;
         if (sg + 2 <= nseg - 1) HIPCHK(hipStreamWaitEvent(s, g->ev_adj[p], 0));
+        smp.cut();
         // all b-a steps are replayed: the last one only to store the slope f(y_{b-1}); kept
         // segments are not (a member that keeps more segments than the group recomputes some of
         // its kept ones, into their own buffers, with the same values)
@@ -912,6 +948,7 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
             for (int j = 0; j < b - a; j++) smp.fwd_launch(s, [&] { group_forward(g, MODE_REC, a + j, a, s); });
         HIPCHK(hipEventRecord(g->ev_rec[p], s));
         HIPCHK(hipStreamWaitEvent(side, g->ev_rec[p], 0));
+        smp.cut();
         for (int i = b; i > a; i--) smp.adj_launch(side, [&] { group_adjoint(g, i, a, side); });
         HIPCHK(hipEventRecord(g->ev_adj[p], side));
         HIPCHK(hipGetLastError());
@@ -1854,7 +1891,19 @@ int fkadj_plan_grad(fkadj_plan *P, const double *const *E, double *const *grad, 
             HIPCHK(hipEventRecord(P->gs[j]->ev_lane, s));
             HIPCHK(hipStreamWaitEvent(lanes[j % lanes.size()], P->gs[j]->ev_lane, 0));
         }
-        for (size_t j = 0; j < ng; j++) HIPCHK(hipGraphLaunch(P->execs[j], lanes[j % lanes.size()]));
+        // each graph launched from a host thread of its own: a launch submits its nodes one by one, so
+        // from one thread the second group's graph would start only after the first one's nodes
+        std::vector<hipError_t> lrc(ng, hipSuccess);
+        std::vector<std::thread> threads;
+        try {
+            for (size_t j = 1; j < ng; j++)
+                threads.emplace_back([&, j] { lrc[j] = hipGraphLaunch(P->execs[j], lanes[j % lanes.size()]); });
+        } catch (...) {
+            for (size_t j = threads.size() + 1; j < ng; j++) lrc[j] = hipGraphLaunch(P->execs[j], lanes[j % lanes.size()]);
+        }
+        lrc[0] = hipGraphLaunch(P->execs[0], lanes[0]);
+        for (auto &t : threads) t.join();
+        for (size_t j = 0; j < ng; j++) HIPCHK(lrc[j]);
         for (size_t j = 0; j < ng; j++) {
             HIPCHK(hipEventRecord(P->gs[j]->ev_lane, lanes[j % lanes.size()]));
             HIPCHK(hipStreamWaitEvent(s, P->gs[j]->ev_lane, 0));
@@ -2145,16 +2194,25 @@ int server_round(fkadj_server *S)
     }
     for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
     int64_t fcells = 0, acells = 0;   // cells of this round's forward-kind and adjoint launches
+    std::map<int, int64_t> fpc, apc;  // the same per pitch
     for (int k = 0; k < S->max_slots; k++)
         if (S->slot_req[k]) {
-            const int64_t cells = (int64_t)S->slot_req[k]->I->rows * S->slot_req[k]->I->pitch;
-            (S->slot_req[k]->phase != PH_ADJ ? fcells : acells) += cells;
+            const fkadj_instance *I = S->slot_req[k]->I;
+            const int64_t cells = (int64_t)I->rows * I->pitch;
+            const bool fw = S->slot_req[k]->phase != PH_ADJ;
+            (fw ? fcells : acells) += cells;
+            (fw ? fpc : apc)[I->pitch] += cells;
         }
-    // large rounds take the row-rolling kernels for the slots whose pitch they support (one launch
-    // per pitch; the same results as the tiled kernels)
+    // the slots of a pitch that the row-rolling kernels support take them when that pitch alone has
+    // 4M cells or more in the launch (one launch per such pitch; the same results as the tiled
+    // kernels); the other slots share one tiled launch.  A rolling launch fills the GPU from 4M cells
+    // on (256 CUs x 16 waves x 1024 cells per wave of a 16-row strip); a smaller one would leave
+    // part of it idle while the round's launches run one after the other on the server's stream.
     const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
-    const bool rollf = rmode == 2 || (rmode == 1 && fcells >= kRollCells);
-    const bool rolla = rmode == 2 || (rmode == 1 && acells >= kRollCells);
+    auto roll_for = [&](const std::map<int, int64_t> &pc, int pitch) {
+        if (!roll_pitch_supported(pitch)) return false;
+        return rmode == 2 || (rmode == 1 && pc.at(pitch) >= 2 * kRollCells);
+    };
     std::map<int, LaunchMap> rfm, ram;
     auto rmap = [](std::map<int, LaunchMap> &mm, int pitch) -> LaunchMap & {
         auto it = mm.find(pitch);
@@ -2172,14 +2230,14 @@ int server_round(fkadj_server *S)
         if (r->phase == PH_ADJ) {
             ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
             const int ac = adj_cfg_of(I, acells);
-            if (rolla && roll_pitch_supported(I->pitch))
+            if (roll_for(apc, I->pitch))
                 launch_map_add(rmap(ram, I->pitch), k, (I->rows + hs - 1) / hs);
             else
                 launch_map_add(am[ac], k, tiles_of(I->rows, I->pitch, ac));
         } else {
             fc.c[k] = r->phase == PH_FWD ? SlotCursor{MODE_FWD, r->step, 0, 0}
                                          : SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0};
-            if (rollf && roll_pitch_supported(I->pitch))
+            if (roll_for(fpc, I->pitch))
                 launch_map_add(rmap(rfm, I->pitch), k, (I->rows + hs - 1) / hs);
             else
                 launch_map_add(fm[f], k, tiles_of(I->rows, I->pitch, f));

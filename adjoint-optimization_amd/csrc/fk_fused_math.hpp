@@ -47,6 +47,15 @@ __device__ inline float stencil5(const LapW &w, float top, float bot, float lft,
     return fmaf(w.t, top, fmaf(w.b, bot, fmaf(w.l, lft, fmaf(w.r, rgt, -4.0f * ctr))));
 }
 
+// f(y) + E gh from the reaction terms (fu, fv, fw) and the three stencils (s0, s1, s2)
+__device__ inline void rhs_combine(const Model32 &m, const LapW &w, float E, float fu, float fv, float fw, float s0,
+                                   float s1, float s2, float *f)
+{
+    f[0] = fmaf(m.lc[0], s0, fmaf(E, w.gh, fu));
+    f[1] = fmaf(m.lc[1], s1, fv);
+    f[2] = fmaf(m.lc[2], s2, fw);
+}
+
 // f(y) + E gh at an active cell from its value and its four neighbours' values, per variable
 __device__ inline void rhs_vals(const Model32 &m, const LapW &w, const float *yc, const float *top, const float *bot,
                                 const float *lft, const float *rgt, float E, float *f)
@@ -54,9 +63,9 @@ __device__ inline void rhs_vals(const Model32 &m, const LapW &w, const float *yc
     Gates32 g = gates32(m, yc[0]);
     float fu, fv, fw;
     fion32(m, g, yc[0], yc[1], yc[2], fu, fv, fw);
-    f[0] = fmaf(m.lc[0], stencil5(w, top[0], bot[0], lft[0], rgt[0], yc[0]), fmaf(E, w.gh, fu));
-    f[1] = fmaf(m.lc[1], stencil5(w, top[1], bot[1], lft[1], rgt[1], yc[1]), fv);
-    f[2] = fmaf(m.lc[2], stencil5(w, top[2], bot[2], lft[2], rgt[2], yc[2]), fw);
+    rhs_combine(m, w, E, fu, fv, fw, stencil5(w, top[0], bot[0], lft[0], rgt[0], yc[0]),
+                stencil5(w, top[1], bot[1], lft[1], rgt[1], yc[1]), stencil5(w, top[2], bot[2], lft[2], rgt[2], yc[2]),
+                f);
 }
 
 __device__ inline void jtz(const Jac32 &J, const float *z, float *o)

@@ -41,16 +41,15 @@ template <int RY, int RX>
 __device__ inline void rhs_lds(const Model32 &m, const float (*s)[RY][RX], const LapW &w, int ly, int lx, float E,
                                float *f, float *yc)
 {
-    float top[3], bot[3], lft[3], rgt[3];
+    // the reaction first, then the stencils straight from LDS (fewer values live at once than
+    // gathering the neighbours first; the same expressions as rhs_vals)
 #pragma unroll
-    for (int v = 0; v < 3; v++) {
-        yc[v] = s[v][ly][lx];
-        top[v] = s[v][ly - 1][lx];
-        bot[v] = s[v][ly + 1][lx];
-        lft[v] = s[v][ly][lx - 1];
-        rgt[v] = s[v][ly][lx + 1];
-    }
-    rhs_vals(m, w, yc, top, bot, lft, rgt, E, f);
+    for (int v = 0; v < 3; v++) yc[v] = s[v][ly][lx];
+    Gates32 g = gates32(m, yc[0]);
+    float fu, fv, fw;
+    fion32(m, g, yc[0], yc[1], yc[2], fu, fv, fw);
+    rhs_combine(m, w, E, fu, fv, fw, stencil<RY, RX>(s, 0, ly, lx, w, yc[0]), stencil<RY, RX>(s, 1, ly, lx, w, yc[1]),
+                stencil<RY, RX>(s, 2, ly, lx, w, yc[2]), f);
 }
 
 // L^T z at an active cell from the LDS tile (missing neighbours have weight 0)

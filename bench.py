@@ -626,7 +626,7 @@ def main():
     total = time.perf_counter() - t_start
     value = B * args.steps / total
     # roofline pass: one more batched evaluation as one group (kernels alone on the GPU, so a launch's
-    # HIP events time that kernel), sampling one launch in eight
+    # HIP events time that kernel), timing windows of 8 back-to-back launches, one in every 64
     groups = fkadj.set_batch_groups(1)
     fkadj.set_profiling(True)
     ts = time.perf_counter()
@@ -680,7 +680,8 @@ def main():
     # roofline of the dominant kernel from the in-run HIP-event samples
     cells = B * N * ((N + 63) // 64 * 64)
     roof = roofline_of(timings, cells, ckpt, nt, args.steps, total)   # cells: all instances of a step
-    roof["source"] = (f"one extra batched evaluation run as one group with HIP events on one launch in eight "
+    roof["source"] = (f"one extra batched evaluation run as one group, HIP events around windows of 8 back-to-back "
+                      f"launches (one window per 64 launches) "
                       f"({roof_ms:.0f} ms); the timed steps run {groups} groups without events")
     if copy:
         roof["copy_peak_GBs"] = copy["GBs"]

@@ -230,9 +230,11 @@ typedef struct fkadj_timing {
     int64_t cell_steps;      /* active+padding cells x steps processed per sweep */
 } fkadj_timing;
 
-/* Enables per-launch HIP events on the next calls (adds small overhead); 0 disables.  The events
- * time each launch on its own stream: the groups of a split batch call (fkadj_set_batch_groups) run
- * at the same time, so their launches overlap; set 1 group to time kernels alone. */
+/* Enables launch timing with HIP events on the next calls (adds small overhead); 0 disables.  One
+ * window of 8 back-to-back launches of one kind (forward-kind or adjoint) in every 64 is bracketed by
+ * two events on its own stream; the mean launch duration is the windows' time over their launches.
+ * The groups of a split batch call (fkadj_set_batch_groups) run at the same time, so their launches
+ * overlap; set 1 group to time kernels alone. */
 int fkadj_set_profiling(int on);
 int fkadj_last_timing(fkadj_timing *t);
 /* fkadj_batch_grad calls over 2M cells or more run their members as up to n groups (contiguous

@@ -14,9 +14,10 @@
 * C5 -- 2048^2, 400 holes, four interacting spirals, T = 500 ms (25 000 steps), 500 amplitudes: the FP32
   gradient within 5e-3 of the FP64 path and J within 1e-4;
 * the C5 horizon on a 256^2 spiral (T = 500 ms, 500 amplitudes): the FP64 GPU path within 1e-9 of the
-  oracle's gradient stored by tests/golden/make_horizon_oracle.py over all 25 000 steps, and the FP32
-  path within 8 times the error that FP32 storage of the state alone causes (the same oracle rounding
-  its trajectory to FP32 after every step).
+  oracle's gradient stored by tests/golden/make_horizon_oracle.py over all 25 000 steps (measured
+  1.1e-14), and the FP32 path within 2 times the error that FP32 storage of the state alone causes (the
+  same oracle rounding its trajectory to FP32 after every step; measured 2.51e-4 against a floor of
+  2.43e-4).
 
 Reference path: fk_grad_Et_comp, src/cuda/fk_grad_Et_mono_PARAMETERS.cu:1569-1938.
 """
@@ -248,4 +249,4 @@ def test_horizon_fp64_against_oracle_and_fp32_against_storage_floor():
     print(f"[horizon] FP64 vs oracle {e64:.2e}; FP32 vs oracle {e32:.2e} = {e32 / floor:.1f} x the FP32-state "
           f"floor {floor:.2e}", flush=True)
     assert e64 < 1e-9 and abs(got[fkadj.F64][1] / ref["Lcost"] - 1) < 1e-10
-    assert e32 < 8 * floor and abs(got[fkadj.F32][1] / ref["Lcost"] - 1) < 1e-4
+    assert e32 < 2 * floor and abs(got[fkadj.F32][1] / ref["Lcost"] - 1) < 1e-4

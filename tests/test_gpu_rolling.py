@@ -114,3 +114,20 @@ def test_served_requests_on_rolling_rounds_equal_alone():
     _same(alone, res)
     for inst, _ in pairs:
         inst.close()
+
+
+def test_mixed_pitch_batch_rolls_per_pitch():
+    """A batch of pitches 256, 384 (both rolled, one launch each) and 320 (not supported by the rolling
+    kernels: tiled) in one group, members given out of pitch order: every result equals the instance
+    alone on the tiled kernels."""
+    _print_case("roll_mixed_pitch", n=5)
+    specs = [(40, 300, 40, 4, 10, 81), (44, 250, 40, 4, 10, 82), (36, 380, 40, 4, 10, 83), (52, 256, 40, 4, 10, 84),
+             (30, 330, 40, 4, 10, 85)]
+    pairs = [_instance(H, W, nt, K, ck, seed) for (H, W, nt, K, ck, seed) in specs]
+    insts = [p[0] for p in pairs]
+    Es = [p[1] for p in pairs]
+    alone = [_grad(0, 16, [i], [e])[0] for i, e in zip(insts, Es)]
+    mixed = _grad(2, 16, insts, Es)
+    _same(alone, mixed)
+    for i in insts:
+        i.close()

@@ -30,14 +30,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "adjoint-optimization_amd"))
 
 HBM_PEAK_GBS = 8000.0
-# Measured per-cell figures of the kernels of the benchmark's launches (16 x 512^2 split into two groups
-# of 8, 2 097 152 cells per dispatch), from rocprofv3 --pmc runs summarised in
-# profiles/roll_c2x16_counters.md: HBM bytes = FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors a 2 GiB
-# copy gives those counters), VALU lane-instructions = SQ_INSTS_VALU x 64 / cells.  The forward figure
-# is the mean of the forward-sweep and recompute launches.
-PROFILE = "profiles/roll_c2x16_counters.md"
-HBM_BYTES_PER_CELL = {"adj": 111.4, "fwd": 40.0}
-VALU_LANE_INSTR_PER_CELL = {"adj": 675.0, "fwd": 378.0}
+# Measured per-cell figures of the two kernel families from rocprofv3 --pmc runs: HBM bytes =
+# FETCH_SIZE x 2.0 + WRITE_SIZE x 1.0 (the factors a 2 GiB copy gives those counters), VALU
+# lane-instructions = SQ_INSTS_VALU x 64 / cells; forward figures are the mean of the forward-sweep and
+# recompute launches.  Row-rolling kernels on the benchmark's launches (16 x 512^2 in two groups of 8,
+# 2 097 152 cells per dispatch); tiled kernels (64 x 16 cells, 512 threads) on one 2048^2 instance
+# (4 194 304 cells per dispatch, the C5 launch shape).
+FAMILIES = {
+    "roll": {"profile": "profiles/roll_c2x16_counters.md", "bytes": {"adj": 111.4, "fwd": 40.0},
+             "valu": {"adj": 675.0, "fwd": 378.0},
+             "names": {"adj": "k_roll_adj<NW> (adjoint step, row-rolling)",
+                       "fwd": "k_roll_fwd<NW> (forward/recompute step, row-rolling)"}},
+    "tiled": {"profile": "profiles/fused_c5_2048_counters.md", "bytes": {"adj": 78.8, "fwd": 33.2},
+              "valu": {"adj": 824.0, "fwd": 502.0},
+              "names": {"adj": "k_fused_adj<16, 512, 1> (adjoint step, 64x16 tiles)",
+                        "fwd": "k_fused_fwd<16, 512, 8, 1> (forward/recompute step, 64x16 tiles)"}},
+}
 # the issue peak is one wave64 VALU instruction per CU per clock: 256 CUs x 2.4 GHz (non-packed FP32)
 VALU_PEAK_G_WAVE_INSTR_S = 256 * 2.4
 
@@ -174,8 +182,10 @@ def kept_budgets(shapes, nt, ckpt, spare):
     return k, [k * b for b in seg]
 
 
-def roofline_of(timings, cells, ckpt, nt, evals, seconds):
-    """Dominant fused kernel's algorithmic-bytes rate from the in-run HIP-event samples."""
+def roofline_of(timings, cells, ckpt, nt, evals, seconds, family="roll"):
+    """Dominant fused kernel's algorithmic-bytes rate from the in-run HIP-event samples; `family` names
+    the kernels the launches ran (row-rolling or tiled), whose counter profile gives traffic and VALU."""
+    F = FAMILIES[family]
     fwd_ms = float(np.mean([t["fwd_kernel_ms"] for t in timings]))
     adj_ms = float(np.mean([t["adj_kernel_ms"] for t in timings]))
     nf = timings[-1]["fwd_launches"]
@@ -183,21 +193,22 @@ def roofline_of(timings, cells, ckpt, nt, evals, seconds):
     fwd_bytes = 7 * 4 * cells
     adj_bytes = 13 * 4 * cells
     if adj_ms * na >= fwd_ms * nf:
-        dom, dbytes, dms, dkey = "k_roll_adj<8> (adjoint step, row-rolling)", adj_bytes, adj_ms, "adj"
+        dbytes, dms, dkey = adj_bytes, adj_ms, "adj"
     else:
-        dom, dbytes, dms, dkey = "k_roll_fwd<8> (forward/recompute step, row-rolling)", fwd_bytes, fwd_ms, "fwd"
+        dbytes, dms, dkey = fwd_bytes, fwd_ms, "fwd"
+    dom = F["names"][dkey]
     achieved = dbytes / (dms * 1e-3) / 1e9
     # compulsory words of the launches of one evaluation (kept segments are not recomputed, so the
     # forward launches count nt sweep steps plus the recomputed ones)
     eval_bytes = 4 * cells * (7 * nf + 13 * na)
 
     def valu(key, ms):
-        rate = VALU_LANE_INSTR_PER_CELL[key] / 64 * cells / (ms * 1e-3) / 1e9
-        return {"lane_instr_per_cell": VALU_LANE_INSTR_PER_CELL[key], "achieved": rate,
+        rate = F["valu"][key] / 64 * cells / (ms * 1e-3) / 1e9
+        return {"lane_instr_per_cell": F["valu"][key], "achieved": rate,
                 "peak": VALU_PEAK_G_WAVE_INSTR_S, "unit": "G wave-instr/s", "frac": rate / VALU_PEAK_G_WAVE_INSTR_S}
     return {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS, "traffic": HBM_BYTES_PER_CELL[dkey] * cells,
-            "traffic_GBs": HBM_BYTES_PER_CELL[dkey] * cells / (dms * 1e-3) / 1e9, "traffic_source": PROFILE,
+            "frac": achieved / HBM_PEAK_GBS, "traffic": F["bytes"][dkey] * cells,
+            "traffic_GBs": F["bytes"][dkey] * cells / (dms * 1e-3) / 1e9, "traffic_source": F["profile"],
             "kernel": dom,
             "kernel_ms": dms, "bytes_per_launch": dbytes,
             "fwd": {"ms": fwd_ms, "GBs": fwd_bytes / (fwd_ms * 1e-3) / 1e9, "launches": nf},
@@ -342,7 +353,7 @@ def run_c5(args):
                       "phase_singularities": ps, "footprint_GB_f32": fp32_bytes / 1e9, "kept_segments": kept},
            "plain_f64": plain, "rel_err_vs_f64": rel, "grad_rel_err_by_horizon": horizon,
            "speedup_vs_plain": (args.steps / total) / plain["value"] if plain else None,
-           "roofline": roofline_of(timings, cells, ckpt, nt, args.steps, total)}
+           "roofline": roofline_of(timings, cells, ckpt, nt, args.steps, total, family="tiled")}
     print(json.dumps(out), flush=True)
 
 
@@ -679,7 +690,11 @@ def main():
 
     # roofline of the dominant kernel from the in-run HIP-event samples
     cells = B * N * ((N + 63) // 64 * 64)
-    roof = roofline_of(timings, cells, ckpt, nt, args.steps, total)   # cells: all instances of a step
+    # the roofline pass runs all instances as one group: the row-rolling kernels take it when its pitch
+    # is one they support and the group holds 2M cells or more (the library's rule), else the tiled ones
+    pitch = (N + 63) // 64 * 64
+    family = "roll" if pitch in (256, 384, 512, 768, 1024) and cells >= (2 << 20) else "tiled"
+    roof = roofline_of(timings, cells, ckpt, nt, args.steps, total, family)   # cells: all instances of a step
     roof["source"] = (f"one extra batched evaluation run as one group, HIP events around windows of 8 back-to-back "
                       f"launches (one window per 64 launches) "
                       f"({roof_ms:.0f} ms); the timed steps run {groups} groups without events")

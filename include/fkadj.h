@@ -247,13 +247,14 @@ int fkadj_last_timing(fkadj_timing *t);
  * returns the previous value.  No reference counterpart. */
 int fkadj_set_batch_groups(int n);
 
-/* FP32 groups (a fkadj_grad, a batch call or plan, or one group of a split batch) whose members
- * share one pitch of 256, 384, 512, 768 or 1024 cells can run their forward-kind and adjoint steps
- * with row-rolling kernels: one workgroup per strip of strip_rows rows, as wide as the pitch, the
- * four RK stages of a step skewed by one row, so no halo is recomputed across columns.  Results
- * equal the tiled kernels' bit for bit.  mode 0: never; 1: groups of 2M cells or more (default); 2:
- * every group that can.  strip_rows: a multiple of 8 (8..256; default 16); groups whose members'
- * adjoint tiles do not divide it run tiled.  Applies to groups built after the call (plans keep their
+/* In an FP32 group (a fkadj_grad, a batch call or plan, or one group of a split batch), the members
+ * of a pitch of 256, 384, 512, 768 or 1024 cells can run their forward-kind and adjoint steps with
+ * row-rolling kernels: one workgroup per strip of strip_rows rows, as wide as the pitch, the four RK
+ * stages of a step skewed by one row, so no halo is recomputed across columns; one rolling launch per
+ * such pitch, the other members in the tiled launches of the same step.  Results equal the tiled
+ * kernels' bit for bit.  mode 0: never; 1 (default): the pitches with 2M cells or more in the group,
+ * and in a server round the pitches with 4M cells or more; 2: every such pitch.  strip_rows: a
+ * multiple of 8 (8..256; default 16).  Applies to groups built after the call (plans keep their
  * kernels).  Process-wide; returns the previous mode.  No reference counterpart. */
 int fkadj_set_rolling(int mode, int strip_rows);
 

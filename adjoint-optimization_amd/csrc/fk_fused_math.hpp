@@ -11,9 +11,11 @@
 
 namespace fk {
 
-// Per-cell stencil weights decoded once from the cell code.  Hole and out-of-grid cells hold 0, so
-// the mirror Laplacian (a missing neighbour takes the opposite one's value) is
-// sum_w = (1+[b missing]) a + (1+[a missing]) b per axis, and L^T uses its 0/1/2 weights directly.
+// Per-cell stencil weights decoded once from the cell code.  The mirror Laplacian (a missing
+// neighbour takes the opposite one's value; a cell never misses both neighbours on one axis) is
+// sum_w = [a present](1+[b missing]) a + [b present](1+[a missing]) b per axis: weight 0 for a missing
+// neighbour, 2 for its mirror partner, so the values held at hole and out-of-grid cells never enter
+// an active cell's stencil.  L^T uses its 0/1/2 weights directly.
 struct LapW {
     float t, b, l, r;   // weights of the top (i-1), bottom (i+1), left (j-1), right (j+1) values
     float gh;           // 2 Du/h * ghost_lr
@@ -22,10 +24,10 @@ struct LapW {
 __device__ inline LapW lap_weights(const Model32 &m, uint32_t c)
 {
     LapW w;
-    w.t = (c & C_BOT_MISS) ? 2.0f : 1.0f;
-    w.b = (c & C_TOP_MISS) ? 2.0f : 1.0f;
-    w.l = (c & C_RIGHT_MISS) ? 2.0f : 1.0f;
-    w.r = (c & C_LEFT_MISS) ? 2.0f : 1.0f;
+    w.t = (c & C_TOP_MISS) ? 0.0f : ((c & C_BOT_MISS) ? 2.0f : 1.0f);
+    w.b = (c & C_BOT_MISS) ? 0.0f : ((c & C_TOP_MISS) ? 2.0f : 1.0f);
+    w.l = (c & C_LEFT_MISS) ? 0.0f : ((c & C_RIGHT_MISS) ? 2.0f : 1.0f);
+    w.r = (c & C_RIGHT_MISS) ? 0.0f : ((c & C_LEFT_MISS) ? 2.0f : 1.0f);
     w.gh = m.ghc * (float)code_gh(c);
     return w;
 }

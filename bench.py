@@ -507,6 +507,11 @@ def run_c4(args):
     makespan = max(done_t)
     srv.close()
     lat = np.array([1e3 * (done_t[r] - arrivals[r]) for r in range(R)])
+    # the rate between the first and the last quarter of completions: the server with its slots full,
+    # without the start (slots filling) and the tail (the last requests running with few others)
+    ends = np.sort(np.asarray(done_t))
+    q1, q3 = R // 4, (3 * R) // 4
+    mid_rate = (q3 - q1) / (ends[q3] - ends[q1]) if ends[q3] > ends[q1] else None
     same = bool(np.array_equal(results[0][0], ref0[0]) and results[0][1] == ref0[1])
     cell_steps = sum(insts[w].nrows * ((insts[w].ncols + 63) // 64 * 64) * insts[w].nt for w in which)
     out = {"metric": f"served gradient evals/s (Poisson {rate:g} req/s) with p50/p99 latency", "value": R / makespan,
@@ -515,6 +520,7 @@ def run_c4(args):
            "config": {"workload": f"C4: {R} requests on {P} resident instances, grids 256..1024 step 64, "
                                   f"T 50..200 ms, Poisson {rate:g} req/s, budget 200 GB, up to 64 slots"},
            "offered_req_s": rate, "achieved_req_s": R / makespan, "makespan_s": makespan,
+           "mid_stream_req_s": mid_rate,
            "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                           "min": float(lat.min()), "max": float(lat.max())},
            "request0_equals_alone": same,
@@ -780,6 +786,11 @@ def served_leg(masks, states, A, T, K_amp, ckpt, adj_tile, rate, seed=7):
     for inst in insts:
         inst.close()
     lat = np.array([1e3 * (done_t[r] - arrivals[r]) for r in range(R)])
+    # the rate between the first and the last quarter of completions: the server with its slots full,
+    # without the start (slots filling) and the tail (the last requests running with few others)
+    ends = np.sort(np.asarray(done_t))
+    q1, q3 = R // 4, (3 * R) // 4
+    mid_rate = (q3 - q1) / (ends[q3] - ends[q1]) if ends[q3] > ends[q1] else None
     return {"requests": R, "offered_req_s": rate, "achieved_req_s": R / makespan, "makespan_s": makespan,
             "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                            "min": float(lat.min()), "max": float(lat.max())},

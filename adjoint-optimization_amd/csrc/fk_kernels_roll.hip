@@ -57,14 +57,12 @@ __device__ inline void pin_model(Model32 &m)
     m.i_tau_r = pin_v(m.i_tau_r);
 }
 
-// Lap weights of a row cell packed one per byte (top, bottom, left, right; 0 for a missing neighbour,
-// 2 for its mirror partner, else 1: lap_weights), 0 for an inactive cell: one word per row instead of
-// four registers, unpacked by byte conversions.
+// Lap weights of a row cell packed one per byte (top, bottom, left, right; 1 or 2), 0 for an
+// inactive cell: one word per row instead of four registers, unpacked by byte conversions.
 __device__ inline uint32_t pack_lap(uint32_t c)
 {
     if (!(c & C_ACTIVE)) return 0u;
-    const uint32_t tm = (c >> 1) & 1u, bm = (c >> 2) & 1u, lm = (c >> 3) & 1u, rm = (c >> 4) & 1u;
-    return ((1u - tm) << bm) | (((1u - bm) << tm) << 8) | (((1u - lm) << rm) << 16) | (((1u - rm) << lm) << 24);
+    return 0x01010101u + ((c >> 2) & 1u) + (((c >> 1) & 1u) << 8) + (((c >> 4) & 1u) << 16) + (((c >> 3) & 1u) << 24);
 }
 // L^T weights packed one per byte (0, 1 or 2), 0 for an inactive cell (an active cell has a
 // neighbour on each axis, so its word is not 0).
@@ -169,12 +167,10 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     F.gh[qa] = F.m.ghc * (float)code_gh(F.cn);
     F.load(t + 2);
 
-    // stage 1 on row t.  Every lane evaluates its cell: an inactive or out-of-grid cell gets finite
-    // values from its zero state that no active cell reads (weight 0 in every stencil), so no lane
-    // waits for a zero to be written.
+    // stage 1 on row t
     {
-        float f[3];
-        {
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if ((unsigned)t < (unsigned)F.rows && F.w[q0] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(0, p0, v, -1); rt[v] = F.L.at(0, p0, v, 1); }
@@ -192,8 +188,9 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     // stage 2 on row t-1: Y2 of rows t-2 (above), t-1 (left, right) from the ring, t (below) in registers
     if (t >= F.tb + 2) {
-        float f[3];
-        {
+        const int row = t - 1;
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if ((unsigned)row < (unsigned)F.rows && F.w[qm1] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(1, p1, v, -1); rt[v] = F.L.at(1, p1, v, 1); }
@@ -207,8 +204,9 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     // stage 3 on row t-2
     if (t >= F.tb + 4) {
-        float f[3];
-        {
+        const int row = t - 2;
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if ((unsigned)row < (unsigned)F.rows && F.w[qm2] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(2, p0, v, -1); rt[v] = F.L.at(2, p0, v, 1); }
@@ -223,8 +221,8 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     // stage 4 on row t-3 and the new state
     if (t >= F.tb + 6) {
         const int row = t - 3;
-        float f[3];
-        {
+        float f[3] = {0.0f, 0.0f, 0.0f};
+        if (F.w[qm3] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(3, p1, v, -1); rt[v] = F.L.at(3, p1, v, 1); }

@@ -13,6 +13,9 @@
 // evaluated twice.  Row values that neighbouring lanes read go through LDS in two-row rings (row
 // parity); one barrier per iteration orders them.  A lane reads the row above from the ring slot
 // that the current row's values overwrite afterwards, in its own column only, so two slots suffice.
+// Odd strips roll upwards (the same code with the rolling row order mirrored): the halo rows two
+// neighbouring strips both read then reach both of them at about the same time, at the start or at
+// the end of their runs, so the second read is served by L2 (the neighbours run on one XCD).
 //
 // Every cell's arithmetic is the tiled kernels' (fk_fused_math.hpp, same expressions, same fused
 // multiply-adds), so states, slopes and adjoint fields are bit for bit those of the tiled kernels,
@@ -134,9 +137,16 @@ struct RollFwd {
     float Y2[4][3], Y3[4][3], Y4[4][3];
     float yn[3];          // arriving row (t+1 at iteration t)
     uint32_t cn;
+    int flip;             // y0 + y1 - 1: grid row of rolling row t in a strip rolled upwards
 
-    __device__ void load(int row)
+    // grid row of rolling row t: strips roll down (DIR = 1) or up (DIR = -1)
+    template <int DIR>
+    __device__ int prow(int t) const { return DIR > 0 ? t : flip - t; }
+
+    template <int DIR>
+    __device__ void load(int t)
     {
+        const int row = prow<DIR>(t);
         if ((unsigned)row < (unsigned)rows) {
             ld3<P>(yin, row, n, c4, yn);
             cn = ld_code<P>(code, row, c2);
@@ -149,7 +159,9 @@ struct RollFwd {
 
 // One iteration: the arriving row t+1 enters the rings, the loads of row t+2 are issued, then
 // stage 1 on row t, stage 2 on t-1, stage 3 on t-2, stage 4 on t-3.  PH = (t - tb) mod RF_U.
-template <int P, int PH>
+// Rows are rolling rows; DIR = -1 rolls upwards (the rolling row before is the grid row below,
+// so the stencils take their top and bottom values from the rolling rows in that order).
+template <int P, int PH, int DIR>
 __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
 {
     const int c = F.c;
@@ -165,20 +177,22 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     F.w[qa] = pack_lap(F.cn);
     F.gh[qa] = F.m.ghc * (float)code_gh(F.cn);
-    F.load(t + 2);
+    F.template load<DIR>(t + 2);
 
     // stage 1 on row t
     {
         float f[3] = {0.0f, 0.0f, 0.0f};
-        if ((unsigned)t < (unsigned)F.rows && F.w[q0] != 0u) {
+        const int row = F.template prow<DIR>(t);
+        if ((unsigned)row < (unsigned)F.rows && F.w[q0] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(0, p0, v, -1); rt[v] = F.L.at(0, p0, v, 1); }
-            rhs_vals(F.m, unpack_w(F.w[q0], F.gh[q0]), F.y[q0], F.y[qm1], F.y[qa], lf, rt, F.E1, f);
+            rhs_vals(F.m, unpack_w(F.w[q0], F.gh[q0]), F.y[q0], DIR > 0 ? F.y[qm1] : F.y[qa],
+                     DIR > 0 ? F.y[qa] : F.y[qm1], lf, rt, F.E1, f);
         }
         if (F.has_k && t >= F.y0 && t < F.y1) {
-            st3<P>(F.kout, t, F.n, F.c4, f);
-            if (F.has_k2) st3<P>(F.kout2, t, F.n, F.c4, f);
+            st3<P>(F.kout, row, F.n, F.c4, f);
+            if (F.has_k2) st3<P>(F.kout2, row, F.n, F.c4, f);
         }
 #pragma unroll
         for (int v = 0; v < 3; v++) {
@@ -188,13 +202,14 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     // stage 2 on row t-1: Y2 of rows t-2 (above), t-1 (left, right) from the ring, t (below) in registers
     if (t >= F.tb + 2) {
-        const int row = t - 1;
+        const int row = F.template prow<DIR>(t - 1);
         float f[3] = {0.0f, 0.0f, 0.0f};
         if ((unsigned)row < (unsigned)F.rows && F.w[qm1] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(1, p1, v, -1); rt[v] = F.L.at(1, p1, v, 1); }
-            rhs_vals(F.m, unpack_w(F.w[qm1], F.gh[qm1]), F.Y2[rm1], F.Y2[rm2], F.Y2[r0], lf, rt, F.E23, f);
+            rhs_vals(F.m, unpack_w(F.w[qm1], F.gh[qm1]), F.Y2[rm1], DIR > 0 ? F.Y2[rm2] : F.Y2[r0],
+                     DIR > 0 ? F.Y2[r0] : F.Y2[rm2], lf, rt, F.E23, f);
         }
 #pragma unroll
         for (int v = 0; v < 3; v++) {
@@ -204,13 +219,14 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     // stage 3 on row t-2
     if (t >= F.tb + 4) {
-        const int row = t - 2;
+        const int row = F.template prow<DIR>(t - 2);
         float f[3] = {0.0f, 0.0f, 0.0f};
         if ((unsigned)row < (unsigned)F.rows && F.w[qm2] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(2, p0, v, -1); rt[v] = F.L.at(2, p0, v, 1); }
-            rhs_vals(F.m, unpack_w(F.w[qm2], F.gh[qm2]), F.Y3[rm2], F.Y3[rm3], F.Y3[rm1], lf, rt, F.E23, f);
+            rhs_vals(F.m, unpack_w(F.w[qm2], F.gh[qm2]), F.Y3[rm2], DIR > 0 ? F.Y3[rm3] : F.Y3[rm1],
+                     DIR > 0 ? F.Y3[rm1] : F.Y3[rm3], lf, rt, F.E23, f);
         }
 #pragma unroll
         for (int v = 0; v < 3; v++) {
@@ -220,13 +236,14 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
     }
     // stage 4 on row t-3 and the new state
     if (t >= F.tb + 6) {
-        const int row = t - 3;
+        const int row = F.template prow<DIR>(t - 3);
         float f[3] = {0.0f, 0.0f, 0.0f};
         if (F.w[qm3] != 0u) {
             float lf[3], rt[3];
 #pragma unroll
             for (int v = 0; v < 3; v++) { lf[v] = F.L.at(3, p1, v, -1); rt[v] = F.L.at(3, p1, v, 1); }
-            rhs_vals(F.m, unpack_w(F.w[qm3], F.gh[qm3]), F.Y4[rm3], F.Y4[rm4], F.Y4[rm2], lf, rt, F.E4, f);
+            rhs_vals(F.m, unpack_w(F.w[qm3], F.gh[qm3]), F.Y4[rm3], DIR > 0 ? F.Y4[rm4] : F.Y4[rm2],
+                     DIR > 0 ? F.Y4[rm2] : F.Y4[rm4], lf, rt, F.E4, f);
         }
         const bool act = F.w[qm3] != 0u;
         float yo[3];
@@ -243,6 +260,54 @@ __device__ __forceinline__ void roll_fwd_iter(RollFwd<P> &F, int t)
         if (t >= F.tb + 4) F.L.at(3, p0, v, 0) = F.Y4[rm2][v];
     }
     __syncthreads();
+}
+
+template <int P, int DIR>
+__device__ __forceinline__ void roll_fwd_run(RollFwd<P> &F)
+{
+    // rows tb-1 and tb before the first iteration (row tb also for its neighbours), row tb+1 in flight
+    F.template load<DIR>(F.tb - 1);
+    constexpr int q_m1 = 5, q_0 = 0;   // ring slots of rows tb-1 and tb
+#pragma unroll
+    for (int v = 0; v < 3; v++) F.y[q_m1][v] = F.yn[v];
+    F.w[q_m1] = pack_lap(F.cn);
+    F.gh[q_m1] = F.m.ghc * (float)code_gh(F.cn);
+    F.template load<DIR>(F.tb);
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        F.y[q_0][v] = F.yn[v];
+        F.L.at(0, 0, v, 0) = F.yn[v];
+    }
+    F.w[q_0] = pack_lap(F.cn);
+    F.gh[q_0] = F.m.ghc * (float)code_gh(F.cn);
+    F.template load<DIR>(F.tb + 1);
+    __syncthreads();
+    const int te = F.y1 + 3;
+    for (int t0 = F.tb; t0 < te; t0 += RF_U) {
+        roll_fwd_iter<P, 0, DIR>(F, t0);
+        if (t0 + 1 >= te) break;
+        roll_fwd_iter<P, 1, DIR>(F, t0 + 1);
+        if (t0 + 2 >= te) break;
+        roll_fwd_iter<P, 2, DIR>(F, t0 + 2);
+        if (t0 + 3 >= te) break;
+        roll_fwd_iter<P, 3, DIR>(F, t0 + 3);
+        if (t0 + 4 >= te) break;
+        roll_fwd_iter<P, 4, DIR>(F, t0 + 4);
+        if (t0 + 5 >= te) break;
+        roll_fwd_iter<P, 5, DIR>(F, t0 + 5);
+        if (t0 + 6 >= te) break;
+        roll_fwd_iter<P, 6, DIR>(F, t0 + 6);
+        if (t0 + 7 >= te) break;
+        roll_fwd_iter<P, 7, DIR>(F, t0 + 7);
+        if (t0 + 8 >= te) break;
+        roll_fwd_iter<P, 8, DIR>(F, t0 + 8);
+        if (t0 + 9 >= te) break;
+        roll_fwd_iter<P, 9, DIR>(F, t0 + 9);
+        if (t0 + 10 >= te) break;
+        roll_fwd_iter<P, 10, DIR>(F, t0 + 10);
+        if (t0 + 11 >= te) break;
+        roll_fwd_iter<P, 11, DIR>(F, t0 + 11);
+    }
 }
 
 // Forward-kind step (MODE_FWD, MODE_REC, MODE_KEEP) of one strip of Hs rows per workgroup; one lane
@@ -313,49 +378,11 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_fwd(const FusedSlot *__rest
     // zero pads: columns -1 and P
     constexpr int CSF = Rings<P, 4>::CS;
     for (int q = F.c; q < 2 * CSF; q += P) lds[q < CSF ? q : (P + 1) * CSF + q - CSF] = 0.0f;
-    // rows tb-1 and tb before the first iteration (row tb also for its neighbours), row tb+1 in flight
-    F.load(F.tb - 1);
-    constexpr int q_m1 = 5, q_0 = 0;   // ring slots of rows tb-1 and tb
-#pragma unroll
-    for (int v = 0; v < 3; v++) F.y[q_m1][v] = F.yn[v];
-    F.w[q_m1] = pack_lap(F.cn);
-    F.gh[q_m1] = F.m.ghc * (float)code_gh(F.cn);
-    F.load(F.tb);
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-        F.y[q_0][v] = F.yn[v];
-        F.L.at(0, 0, v, 0) = F.yn[v];
-    }
-    F.w[q_0] = pack_lap(F.cn);
-    F.gh[q_0] = F.m.ghc * (float)code_gh(F.cn);
-    F.load(F.tb + 1);
-    __syncthreads();
-    const int te = F.y1 + 3;
-    for (int t0 = F.tb; t0 < te; t0 += RF_U) {
-        roll_fwd_iter<P, 0>(F, t0);
-        if (t0 + 1 >= te) break;
-        roll_fwd_iter<P, 1>(F, t0 + 1);
-        if (t0 + 2 >= te) break;
-        roll_fwd_iter<P, 2>(F, t0 + 2);
-        if (t0 + 3 >= te) break;
-        roll_fwd_iter<P, 3>(F, t0 + 3);
-        if (t0 + 4 >= te) break;
-        roll_fwd_iter<P, 4>(F, t0 + 4);
-        if (t0 + 5 >= te) break;
-        roll_fwd_iter<P, 5>(F, t0 + 5);
-        if (t0 + 6 >= te) break;
-        roll_fwd_iter<P, 6>(F, t0 + 6);
-        if (t0 + 7 >= te) break;
-        roll_fwd_iter<P, 7>(F, t0 + 7);
-        if (t0 + 8 >= te) break;
-        roll_fwd_iter<P, 8>(F, t0 + 8);
-        if (t0 + 9 >= te) break;
-        roll_fwd_iter<P, 9>(F, t0 + 9);
-        if (t0 + 10 >= te) break;
-        roll_fwd_iter<P, 10>(F, t0 + 10);
-        if (t0 + 11 >= te) break;
-        roll_fwd_iter<P, 11>(F, t0 + 11);
-    }
+    F.flip = F.y0 + F.y1 - 1;
+    // odd strips roll upwards: the rows two neighbouring strips both read are then read by both at
+    // about the same time, and the second read finds them in L2
+    if (strip & 1) roll_fwd_run<P, -1>(F);
+    else roll_fwd_run<P, 1>(F);
 }
 
 
@@ -386,11 +413,19 @@ struct RollAdj {
     float zn[3], an[3], bn[3], kan[3], kbn[3];
     uint32_t cn;
     float ar[3];          // y_{i-1} of row t-3 for stage 4
+    int flip;             // y0 + y1 - 1: grid row of rolling row t in a strip rolled upwards
+    FK_LDS double *prs;   // this wavefront's row sums of the current partial block (upward strips)
 
-    __device__ void load(int row)   // the arrivals of iteration `row`
+    // grid row of rolling row t: strips roll down (DIR = 1) or up (DIR = -1)
+    template <int DIR>
+    __device__ int prow(int t) const { return DIR > 0 ? t : flip - t; }
+
+    template <int DIR>
+    __device__ void load(int t)   // the arrivals of iteration t: z of rolling row t+1, the rest of row t
     {
-        if ((unsigned)(row + 1) < (unsigned)rows) {
-            ld3<P>(zin, row + 1, n, c4, zn);
+        const int rz = prow<DIR>(t + 1), row = prow<DIR>(t);
+        if ((unsigned)rz < (unsigned)rows) {
+            ld3<P>(zin, rz, n, c4, zn);
         } else {
             zn[0] = zn[1] = zn[2] = 0.0f;
         }
@@ -408,7 +443,10 @@ struct RollAdj {
     }
 };
 
-template <int P, int PH>
+// Rows are rolling rows (t-1 is the row rolled before t); DIR = -1 rolls a strip upwards, so the
+// rolling row before is the grid row below: every stencil takes its top and bottom values from
+// the rolling rows in that order, with the same expressions as rolling down.
+template <int P, int PH, int DIR>
 __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
 {
     const int c = A.c;
@@ -418,7 +456,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
     constexpr int h0 = PH & 1, h1 = (PH + 1) & 1;   // mod-2 slots of rows t (t-2) and t-1; also LDS parities
     constexpr int p0 = h0, p1 = h1;
     // y_{i-1} of row t-3 for stage 4 (in the grid whenever stage 4 runs)
-    if (t >= A.tb + 6) ld3<P>(A.yA, t - 3, A.n, A.c4, A.ar);
+    if (t >= A.tb + 6) ld3<P>(A.yA, A.template prow<DIR>(t - 3), A.n, A.c4, A.ar);
     // arrivals: z of row t+1 (its ring slot held row t-1, whose neighbours' values were read in the
     // previous iteration), the code of row t
     const float zup[3] = {A.z[qm1][0], A.z[qm1][1], A.z[qm1][2]};
@@ -452,7 +490,8 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         float lt[3];
 #pragma unroll
         for (int v = 0; v < 3; v++)
-            lt[v] = m.lc[v] * stencil5(tw, zup[v], A.z[qa][v], A.X.at(0, p0, v, -1), A.X.at(0, p0, v, 1), A.z[q0][v]);
+            lt[v] = m.lc[v] * stencil5(tw, DIR > 0 ? zup[v] : A.z[qa][v], DIR > 0 ? A.z[qa][v] : zup[v],
+                                       A.X.at(0, p0, v, -1), A.X.at(0, p0, v, 1), A.z[q0][v]);
         A.dz[q0][0] = A.dz[q0][1] = A.dz[q0][2] = 0.0f;
         if (act0) {
             Gates32 g = gates32(m, A.bn[0]);
@@ -467,7 +506,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
             }
         }
     }
-    A.load(t + 1);
+    A.template load<DIR>(t + 1);
     // stage 2 on row t-1: midpoint state yh = y4 + 27/64 dt f(y(t_{i-1} + dt/3)), J(yh)
     float Z2[3] = {0.0f, 0.0f, 0.0f};
     if (t >= A.tb + 2) {
@@ -477,7 +516,9 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
 #pragma unroll
         for (int v = 0; v < 3; v++) {
             zc[v] = A.X.at(2, p1, v, 0);
-            lt[v] = m.lc[v] * stencil5(tw, A.X.at(2, p0, v, 0), Z1[v], A.X.at(2, p1, v, -1), A.X.at(2, p1, v, 1), zc[v]);
+            const float za = A.X.at(2, p0, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, DIR > 0 ? za : Z1[v], DIR > 0 ? Z1[v] : za, A.X.at(2, p1, v, -1),
+                                       A.X.at(2, p1, v, 1), zc[v]);
         }
         if (cc & C_ACTIVE) {
             float yc[3], top[3], lf[3], rt[3], f3[3], yh[3], o[3];
@@ -488,7 +529,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
                 lf[v] = A.X.at(1, p1, v, -1);
                 rt[v] = A.X.at(1, p1, v, 1);
             }
-            rhs_vals(m, lap_weights(m, cc), yc, top, y3, lf, rt, A.E3, f3);
+            rhs_vals(m, lap_weights(m, cc), yc, DIR > 0 ? top : y3, DIR > 0 ? y3 : top, lf, rt, A.E3, f3);
 #pragma unroll
             for (int v = 0; v < 3; v++) yh[v] = A.y4[h1][v] + (float)(27.0 / 64.0) * (f3[v] * dt);
             Gates32 g = gates32(m, yh[0]);
@@ -517,7 +558,9 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
 #pragma unroll
         for (int v = 0; v < 3; v++) {
             zc[v] = A.X.at(3, p0, v, 0);
-            lt[v] = m.lc[v] * stencil5(tw, A.X.at(3, p1, v, 0), Z2[v], A.X.at(3, p0, v, -1), A.X.at(3, p0, v, 1), zc[v]);
+            const float za = A.X.at(3, p1, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, DIR > 0 ? za : Z2[v], DIR > 0 ? Z2[v] : za, A.X.at(3, p0, v, -1),
+                                       A.X.at(3, p0, v, 1), zc[v]);
         }
         if (cc & C_ACTIVE) {
             float o[3];
@@ -535,7 +578,7 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
         for (int v = 0; v < 3; v++) A.X.at(3, p1, v, 0) = Z2[v];
     // stage 4 on row t-3 with J(y_{i-1}); the new z and the row's z.gh sum
     if (t >= A.tb + 6) {
-        const int row = t - 3;
+        const int row = A.template prow<DIR>(t - 3);
         const uint32_t cc = A.cd[qm3];
         const bool act = (cc & C_ACTIVE) != 0;
         const LapW tw = lapT_weights(cc);
@@ -543,7 +586,9 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
 #pragma unroll
         for (int v = 0; v < 3; v++) {
             zc[v] = A.X.at(4, p1, v, 0);
-            lt[v] = m.lc[v] * stencil5(tw, A.X.at(4, p0, v, 0), Z3[v], A.X.at(4, p1, v, -1), A.X.at(4, p1, v, 1), zc[v]);
+            const float za = A.X.at(4, p0, v, 0);
+            lt[v] = m.lc[v] * stencil5(tw, DIR > 0 ? za : Z3[v], DIR > 0 ? Z3[v] : za, A.X.at(4, p1, v, -1),
+                                       A.X.at(4, p1, v, 1), zc[v]);
         }
         if (act) {
             Gates32 g = gates32(m, A.ar[0]);
@@ -554,19 +599,63 @@ __device__ __forceinline__ void roll_adj_iter(RollAdj<P> &A, int t)
             for (int v = 0; v < 3; v++) zo[v] = A.z[qm3][v] + (A.dz[qm3][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
         }
         st3<P>(A.zout, row, A.n, A.c4, zo);
-        const double rs = wave_sum64(zgh_term(m, act, cc, zo[0]));
+        // rows of a wavefront without ghost cells (most of them) add exactly 0.0: no reduction
+        const double term = zgh_term(m, act, cc, zo[0]);
+        const double rs = __builtin_amdgcn_ballot_w64(term != 0.0) ? wave_sum64(term) : 0.0;
         if ((c & 63) == 0) {
             const int j = row % PART_ROWS;
-            if (j == 0) A.tacc = 0.0;
-            A.tacc += rs;
-            if (j == PART_ROWS - 1 || row == A.rows - 1)
-                gmem(A.partial)[(size_t)A.row_i * A.ntiles + (size_t)(row / PART_ROWS) * (P / 64) + (c >> 6)] = A.tacc;
+            auto *dst = gmem(A.partial) + (size_t)A.row_i * A.ntiles + (size_t)(row / PART_ROWS) * (P / 64) + (c >> 6);
+            if (DIR > 0) {
+                if (j == 0) A.tacc = 0.0;
+                A.tacc += rs;
+                if (j == PART_ROWS - 1 || row == A.rows - 1) *dst = A.tacc;
+            } else {
+                // rows of a block arrive from the last one up: keep them, add in row order at row 0
+                A.prs[j] = rs;
+                if (j == 0) {
+                    const int nr = min(PART_ROWS, A.rows - row);
+                    double s = 0.0;
+                    for (int q = 0; q < nr; q++) s += A.prs[q];
+                    *dst = s;
+                }
+            }
         }
     }
     if (t >= A.tb + 4)
 #pragma unroll
         for (int v = 0; v < 3; v++) A.X.at(4, p0, v, 0) = Z3[v];
     __syncthreads();
+}
+
+template <int P, int DIR>
+__device__ __forceinline__ void roll_adj_run(RollAdj<P> &A)
+{
+    // z of rows tb-1 and tb (row tb also for its neighbours' stencils), the arrivals of iteration tb
+    A.template load<DIR>(A.tb - 2);   // z of row tb-1
+#pragma unroll
+    for (int v = 0; v < 3; v++) A.z[5][v] = A.zn[v];
+    A.template load<DIR>(A.tb - 1);   // z of row tb
+#pragma unroll
+    for (int v = 0; v < 3; v++) {
+        A.z[0][v] = A.zn[v];
+        A.X.at(0, 0, v, 0) = A.zn[v];
+    }
+    A.template load<DIR>(A.tb);
+    __syncthreads();
+    const int te = A.y1 + 3;
+    for (int t0 = A.tb; t0 < te; t0 += RA_U) {
+        roll_adj_iter<P, 0, DIR>(A, t0);
+        if (t0 + 1 >= te) break;
+        roll_adj_iter<P, 1, DIR>(A, t0 + 1);
+        if (t0 + 2 >= te) break;
+        roll_adj_iter<P, 2, DIR>(A, t0 + 2);
+        if (t0 + 3 >= te) break;
+        roll_adj_iter<P, 3, DIR>(A, t0 + 3);
+        if (t0 + 4 >= te) break;
+        roll_adj_iter<P, 4, DIR>(A, t0 + 4);
+        if (t0 + 5 >= te) break;
+        roll_adj_iter<P, 5, DIR>(A, t0 + 5);
+    }
 }
 
 // Adjoint step of one strip of Hs rows per workgroup (Hs a multiple of the instance's adjoint tile
@@ -622,38 +711,23 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
     // zero pads: columns -1 and P
     constexpr int CSA = Rings<P, 5>::CS;
     for (int q = A.c; q < 2 * CSA; q += P) lds[q < CSA ? q : (P + 1) * CSA + q - CSA] = 0.0f;
-    // z of rows tb-1 and tb (row tb also for its neighbours' stencils), the arrivals of iteration tb
-    A.load(A.tb - 2);   // z of row tb-1
-#pragma unroll
-    for (int v = 0; v < 3; v++) A.z[5][v] = A.zn[v];
-    A.load(A.tb - 1);   // z of row tb
-#pragma unroll
-    for (int v = 0; v < 3; v++) {
-        A.z[0][v] = A.zn[v];
-        A.X.at(0, 0, v, 0) = A.zn[v];
-    }
-    A.load(A.tb);
-    __syncthreads();
-    const int te = A.y1 + 3;
-    for (int t0 = A.tb; t0 < te; t0 += RA_U) {
-        roll_adj_iter<P, 0>(A, t0);
-        if (t0 + 1 >= te) break;
-        roll_adj_iter<P, 1>(A, t0 + 1);
-        if (t0 + 2 >= te) break;
-        roll_adj_iter<P, 2>(A, t0 + 2);
-        if (t0 + 3 >= te) break;
-        roll_adj_iter<P, 3>(A, t0 + 3);
-        if (t0 + 4 >= te) break;
-        roll_adj_iter<P, 4>(A, t0 + 4);
-        if (t0 + 5 >= te) break;
-        roll_adj_iter<P, 5>(A, t0 + 5);
-    }
+    A.flip = A.y0 + A.y1 - 1;
+    A.prs = (FK_LDS double *)(lds + Rings<P, 5>::FLOATS) + (A.c >> 6) * PART_ROWS;
+    // odd strips roll upwards: the rows two neighbouring strips both read (each one's halo) are then
+    // read by both at about the same time, at the start or at the end of their runs, and the second
+    // read finds them in L2
+    if (strip & 1) roll_adj_run<P, -1>(A);
+    else roll_adj_run<P, 1>(A);
 }
 
 // ------------------------------------------------------------------------------ launchers
-// Dynamic LDS of the two kernels: 4 (forward) and 5 (adjoint) two-row rings of three planes.
+// Dynamic LDS of the two kernels: 4 (forward) and 5 (adjoint) two-row rings of three planes; the
+// adjoint also keeps PART_ROWS row sums per wavefront (FLOATS is even: the sums are 8-byte aligned).
 template <int NW, int NR>
-constexpr size_t roll_lds_bytes() { return sizeof(float) * (size_t)Rings<64 * NW, NR>::FLOATS; }
+constexpr size_t roll_lds_bytes()
+{
+    return sizeof(float) * (size_t)Rings<64 * NW, NR>::FLOATS + (NR == 5 ? sizeof(double) * NW * PART_ROWS : 0);
+}
 
 template <int NW>
 void roll_launch_nw(bool adj, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, int Hs,

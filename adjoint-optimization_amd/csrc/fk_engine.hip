@@ -168,6 +168,8 @@ struct fkadj_instance {
     int nb_plain = 0;     // plain blocks (zgh partial row length of the plain path)
     int nb_part = 0;      // partial row length actually used by the adjoint sweep
     int fwd_cfg = 0, adj_cfg = TILE_16_512;   // fused tile configurations (fwd 0: per group)
+    int fwd_depth = 0, adj_depth = 0;         // time steps per tiled launch (0: built-in rule)
+    int roll_rows = 0;                        // strip rows of the row-rolling kernels (0: library setting)
     int keep = 0;         // FP32: last segments the default context keeps from the forward sweep
     int device = 0;       // GPU the instance lives on (current device at create time)
 
@@ -203,9 +205,12 @@ struct FusedGroupDev {
     hipEvent_t ev_rec[2] = {}, ev_adj[2] = {};
     // one launch map per kernel configuration (index = tile config id); slot k = member k
     LaunchMap fwd[NUM_TILE_CFG], adj[NUM_TILE_CFG];
-    // forward sweep two steps per launch (64x8 tiles, all members) instead of one
+    // forward sweep two steps per launch (64x8 tiles, all members) instead of one; the adjoint
+    // steps likewise (two steps of one segment per launch, 64x8 tiles), where adj_depth is 2
     bool depth2 = false;
     LaunchMap fwd2;
+    int adj_depth = 1;
+    LaunchMap adj2;
     // segments from keep0 on are kept by every member: the forward sweep fills them and the
     // backward pass does not recompute them
     int keep0 = 0;
@@ -214,11 +219,12 @@ struct FusedGroupDev {
     hipEvent_t ev_lane = nullptr;
     // the members of each pitch that the row-rolling kernels support, when the group holds 2M cells
     // or more of that pitch, run them for their forward-kind and adjoint steps (one launch per such
-    // pitch, one workgroup per strip of roll_hs rows); the other members run the tiled kernels
+    // pitch, one workgroup per strip of roll_hs[r] rows); the other members run the tiled kernels
     // (fwd_rest, adj_rest).  The same results either way.  f(y_nt) (MODE_RHS) always takes the tiled
     // maps of all members.
     static constexpr int kMaxRoll = 5;
-    int nroll = 0, roll_hs = 16;
+    int nroll = 0;
+    int roll_hs[kMaxRoll] = {};
     int roll_pitch[kMaxRoll] = {};
     LaunchMap rollmap[kMaxRoll];
     LaunchMap fwd_rest[NUM_TILE_CFG], adj_rest[NUM_TILE_CFG];
@@ -732,11 +738,20 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
     int64_t cells = 0;
     for (auto *C : members) cells += (int64_t)C->I->rows * C->I->pitch;
     launch_map_clear(g->fwd2);
+    launch_map_clear(g->adj2);
     // Two forward steps per launch where one launch costs more than its arithmetic: launches of
     // fewer than 1M cells (those the built-in rule gives 64x8 tiles), members on the built-in
     // forward tile, and a step count and checkpoint interval that pair up.
+    // The members' own choice (fkadj_problem.fwd_depth), else the built-in rule; all members must
+    // agree.  Two adjoint steps per launch only where the members ask for it (fkadj_problem.adj_depth
+    // = 2): its 8-cell halo on 64x8 tiles doubles the stage work of a step, which costs more than
+    // the launch and the Jacobian it saves (lone 512^2 evaluation, T = 200 ms: 270 ms against 231 ms
+    // with one step per launch; 256^2, T = 100 ms: 78 against 77 ms).
     const fkadj_instance *I0 = members[0]->I;
-    g->depth2 = cells < kDepth2Cells && I0->nt % 2 == 0 && I0->K % 2 == 0;
+    const bool small = cells < kDepth2Cells;
+    g->depth2 = (I0->fwd_depth == 2 || (I0->fwd_depth == 0 && small && I0->fwd_cfg == 0)) && I0->nt % 2 == 0 &&
+                I0->K % 2 == 0;
+    g->adj_depth = I0->adj_depth == 2 ? 2 : 1;
     g->keep0 = 0;
     for (size_t k = 0; k < members.size(); k++) {
         fkadj_instance *I = members[k]->I;
@@ -747,19 +762,29 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         const int ac = adj_cfg_of(I, cells);
         launch_map_add(g->adj[ac], (int)k, tiles_of(I->rows, I->pitch, ac));
         launch_map_add(g->fwd2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
-        g->depth2 = g->depth2 && I->fwd_cfg == 0;
+        launch_map_add(g->adj2, (int)k, tiles_of(I->rows, I->pitch, TILE_8_512));
+        if (I->fwd_depth != I0->fwd_depth || I->fwd_cfg != I0->fwd_cfg) g->depth2 = false;
+        if (I->adj_depth != I0->adj_depth) g->adj_depth = 1;
     }
     // row-rolling kernels per pitch they support: every such pitch (mode 2) or those with 2M cells
     // or more in the group (mode 1); strips of hs rows (a multiple of the partial block height)
+    // (the members' own strip rows, fkadj_problem.roll_rows, where all members of the pitch ask for
+    // the same, else the library setting)
     const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
     std::map<int, int64_t> pitch_cells;
-    for (auto *C : members) pitch_cells[C->I->pitch] += (int64_t)C->I->rows * C->I->pitch;
+    std::map<int, int> pitch_rows;   // -1: the members disagree
+    for (auto *C : members) {
+        pitch_cells[C->I->pitch] += (int64_t)C->I->rows * C->I->pitch;
+        auto it = pitch_rows.find(C->I->pitch);
+        if (it == pitch_rows.end()) pitch_rows[C->I->pitch] = C->I->roll_rows;
+        else if (it->second != C->I->roll_rows) it->second = -1;
+    }
     g->nroll = 0;
-    g->roll_hs = hs;
     for (auto &kv : pitch_cells)
         if (roll_pitch_supported(kv.first) && (rmode == 2 || (rmode == 1 && kv.second >= kRollCells)) &&
             g->nroll < FusedGroupDev::kMaxRoll) {
             g->roll_pitch[g->nroll] = kv.first;
+            g->roll_hs[g->nroll] = pitch_rows[kv.first] > 0 ? pitch_rows[kv.first] : hs;
             launch_map_clear(g->rollmap[g->nroll]);
             g->nroll++;
         }
@@ -772,7 +797,7 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         int r = 0;
         while (r < g->nroll && g->roll_pitch[r] != I->pitch) r++;
         if (r < g->nroll) {
-            launch_map_add(g->rollmap[r], (int)k, (I->rows + hs - 1) / hs);
+            launch_map_add(g->rollmap[r], (int)k, (I->rows + g->roll_hs[r] - 1) / g->roll_hs[r]);
         } else {
             const int fc = fwd_cfg_of(I, cells), ac = adj_cfg_of(I, cells);
             launch_map_add(g->fwd_rest[fc], (int)k, tiles_of(I->rows, I->pitch, fc));
@@ -780,6 +805,7 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         }
     }
     if (g->nroll) g->depth2 = false;
+    if (g->nroll) g->adj_depth = 1;
 }
 
 int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
@@ -864,10 +890,10 @@ int groups_build(const std::vector<EvalCtx *> &members_in, hipStream_t s, int ma
     return FKADJ_OK;
 }
 
-Cursors uniform_cursors(int count, int mode, int step, int seg_a)
+Cursors uniform_cursors(int count, int mode, int step, int seg_a, int zpar = 0)
 {
     Cursors c;
-    for (int k = 0; k < MAX_SLOTS; k++) c.c[k] = SlotCursor{k < count ? mode : MODE_IDLE, step, seg_a, 0};
+    for (int k = 0; k < MAX_SLOTS; k++) c.c[k] = SlotCursor{k < count ? mode : MODE_IDLE, step, seg_a, zpar};
     return c;
 }
 
@@ -876,7 +902,8 @@ void group_forward(const FusedGroupDev *g, int mode, int step, int seg_a, hipStr
     const Cursors c = uniform_cursors((int)g->members.size(), mode, step, seg_a);
     if (g->nroll && mode != MODE_RHS) {
         for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd_rest[k], c, s);
-        for (int r = 0; r < g->nroll; r++) roll_launch(false, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs, s);
+        for (int r = 0; r < g->nroll; r++)
+            roll_launch(false, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs[r], s);
         return;
     }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_forward_launch(k, g->d_slots, g->fwd[k], c, s);
@@ -888,15 +915,24 @@ void group_forward2(const FusedGroupDev *g, int mode, int step, int seg_a, hipSt
     fused_forward2_launch(g->d_slots, g->fwd2, c, s);
 }
 
-void group_adjoint(const FusedGroupDev *g, int i, int seg_a, hipStream_t s)
+// One adjoint step (i) of every member, z read from half zpar of the members' zb, written to the other.
+void group_adjoint(const FusedGroupDev *g, int i, int seg_a, int zpar, hipStream_t s)
 {
-    const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a);
+    const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ, i, seg_a, zpar);
     if (g->nroll) {
         for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj_rest[k], c, s);
-        for (int r = 0; r < g->nroll; r++) roll_launch(true, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs, s);
+        for (int r = 0; r < g->nroll; r++)
+            roll_launch(true, g->roll_pitch[r], g->d_slots, g->rollmap[r], c, g->roll_hs[r], s);
         return;
     }
     for (int k = 1; k < NUM_TILE_CFG; k++) fused_adjoint_launch(k, g->d_slots, g->adj[k], c, s);
+}
+
+// Adjoint steps i and i - 1 (both inside the segment starting at seg_a) of every member in one launch.
+void group_adjoint2(const FusedGroupDev *g, int i, int seg_a, int zpar, hipStream_t s)
+{
+    const Cursors c = uniform_cursors((int)g->members.size(), MODE_ADJ2, i, seg_a, zpar);
+    fused_adjoint2_launch(g->d_slots, g->adj2, c, s);
 }
 
 int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
@@ -937,6 +973,7 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
         two_buffers = two_buffers && C->seg_copies == 2;
     }
     hipStream_t side = cells < kTwoStreamCells && two_buffers ? g->side : s;
+    int zpar = 0;
     for (int sg = nseg - 1; sg >= 0; sg--) {
         const int a = sg * K, b = std::min(a + K, nt), p = sg & 1;
         if (sg + 2 <= nseg - 1) HIPCHK(hipStreamWaitEvent(s, g->ev_adj[p], 0));
@@ -949,7 +986,17 @@ int run_fused(const FusedGroupDev *g, hipStream_t s, Sampler &smp)
         HIPCHK(hipEventRecord(g->ev_rec[p], s));
         HIPCHK(hipStreamWaitEvent(side, g->ev_rec[p], 0));
         smp.cut();
-        for (int i = b; i > a; i--) smp.adj_launch(side, [&] { group_adjoint(g, i, a, side); });
+        // the adjoint state alternates between the two halves of zb launch by launch (z_nt in half 0)
+        for (int i = b; i > a;) {
+            if (g->adj_depth == 2 && i - 2 >= a) {
+                smp.adj_launch(side, [&] { group_adjoint2(g, i, a, zpar, side); });
+                i -= 2;
+            } else {
+                smp.adj_launch(side, [&] { group_adjoint(g, i, a, zpar, side); });
+                i -= 1;
+            }
+            zpar ^= 1;
+        }
         HIPCHK(hipEventRecord(g->ev_adj[p], side));
         HIPCHK(hipGetLastError());
     }
@@ -1144,7 +1191,8 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     if ((int64_t)p->nrows * ((p->ncols + 63) / 64 * 64) * 3 * (p->precision == FKADJ_F32 ? 4 : 8) >= (int64_t)1 << 31)
         return FKADJ_ERR_ARG;
     if (p->fwd_tile < 0 || p->fwd_tile >= NUM_TILE_CFG || p->adj_tile < 0 || p->adj_tile >= NUM_TILE_CFG ||
-        p->keep_bytes < 0)
+        p->keep_bytes < 0 || p->fwd_depth < 0 || p->fwd_depth > 2 || p->adj_depth < 0 || p->adj_depth > 2 ||
+        p->roll_rows < 0 || p->roll_rows > 256 || p->roll_rows % 8 != 0)
         return FKADJ_ERR_ARG;
     if (p->pulse_mode == FKADJ_PULSE_KNOTS) {
         if (p->nt + 1 < 4) return FKADJ_ERR_ARG;
@@ -1229,6 +1277,9 @@ int fkadj_create(const fkadj_problem *p, fkadj_instance **out)
     if (I->prec == FKADJ_F32) {
         I->fwd_cfg = p->fwd_tile;   // 0: chosen per group from its total size
         I->adj_cfg = p->adj_tile;   // 0: chosen per launch from its total size
+        I->fwd_depth = p->fwd_depth;
+        I->adj_depth = p->adj_depth;
+        I->roll_rows = p->roll_rows;
         I->nb_part = (I->pitch / FUSED_TX) * ((I->rows + PART_ROWS - 1) / PART_ROWS);   // 64 x 8 partial blocks
     } else {
         I->nb_part = I->nb_plain;
@@ -1973,6 +2024,7 @@ struct ServReq {
     int rc = FKADJ_OK;
     int slot = -1;
     int phase = PH_FWD, step = 0, seg = 0;
+    int zpar = 0;                // half of zb holding the adjoint state (z_nt in half 0)
     bool cancel = false;
     int64_t bytes = 0;
     double *h_out = nullptr;     // pinned: zgh[nt+1], L0
@@ -2228,7 +2280,7 @@ int server_round(fkadj_server *S)
         const fkadj_instance *I = r->I;
         const int K = I->K, f = fwd_cfg_of(I, fcells);
         if (r->phase == PH_ADJ) {
-            ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, 0};
+            ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, r->zpar};
             const int ac = adj_cfg_of(I, acells);
             if (roll_for(apc, I->pitch))
                 launch_map_add(rmap(ram, I->pitch), k, (I->rows + hs - 1) / hs);
@@ -2262,12 +2314,14 @@ int server_round(fkadj_server *S)
                 r->phase = PH_REC;
                 r->seg = (nt + K - 1) / K - 1;
                 r->step = 0;
+                r->zpar = 0;
             }
         } else if (r->phase == PH_REC) {
             const int a = r->seg * K, b = std::min(a + K, nt);
             if (++r->step == b - a) { r->phase = PH_ADJ; r->step = b; }
         } else {
             const int a = r->seg * K;
+            r->zpar ^= 1;
             if (--r->step == a) {
                 if (--r->seg < 0) CHK(server_finish(S, r, false));
                 else { r->phase = PH_REC; r->step = 0; }

@@ -135,16 +135,17 @@ __host__ __device__ inline int seg_buf(const FusedSlot &S, int segi)
 // MODE_FWD: forward sweep (checkpoints + ring); MODE_REC: segment recompute, also stores the
 // stage-1 slopes for the adjoint's Hermite reconstruction; MODE_KEEP: forward-sweep step of a kept
 // segment (states and slopes into the segment's own buffers as MODE_REC, the segment's end state to
-// its checkpoint); MODE_RHS: only f(y_nt) -> kT; MODE_ADJ: adjoint step; MODE_IDLE: the slot's tiles
-// do nothing in this launch.
-enum FusedMode { MODE_IDLE = -1, MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2, MODE_ADJ = 3, MODE_KEEP = 4 };
+// its checkpoint); MODE_RHS: only f(y_nt) -> kT; MODE_ADJ: adjoint step; MODE_ADJ2: two adjoint steps
+// (i and i - 1, inside one segment); MODE_IDLE: the slot's tiles do nothing in this launch.
+enum FusedMode { MODE_IDLE = -1, MODE_FWD = 0, MODE_REC = 1, MODE_RHS = 2, MODE_ADJ = 3, MODE_KEEP = 4, MODE_ADJ2 = 5 };
 
 // Per-slot work of one launch, passed by value (1 KiB of kernel arguments): slots of one launch
 // may be at different steps and phases (continuous batching).  step is the forward step index
 // (MODE_FWD/REC) or the adjoint interval end i (MODE_ADJ); seg_a is the first step of the segment.
 constexpr int MAX_SLOTS = 64;
 struct SlotCursor {
-    int mode, step, seg_a, pad;
+    int mode, step, seg_a;
+    int zpar;   // adjoint launches: the ping-pong half of zb holding the input z (the output goes to the other)
 };
 struct Cursors {
     SlotCursor c[MAX_SLOTS];
@@ -181,6 +182,7 @@ void fused_forward_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &
 void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream);
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,
                           hipStream_t stream);
+void fused_adjoint2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream);
 // Row-rolling kernels (fk_kernels_roll.hip) for instances of one pitch (256, 384, 512, 768 or 1024):
 // one workgroup per strip of Hs rows (launch tiles are strips), Hs a multiple of every member's
 // adjoint tile rows; same results as the tiled kernels bit for bit.

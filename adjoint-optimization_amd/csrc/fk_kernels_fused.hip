@@ -326,8 +326,8 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
                          : (segi + 1 >= S.keep0 ? S.kseg + (size_t)seg_buf(S, segi + 1) * S.K * st
                                                 : S.kfirst + (size_t)((segi + 1) % 3) * st);
     const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext);
-    const float *zin = S.zb + (size_t)((S.nt - i) & 1) * st;
-    auto *zout = gmem(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
+    const float *zin = S.zb + (size_t)(sc.zpar & 1) * st;
+    auto *zout = gmem(S.zb + (size_t)((sc.zpar & 1) ^ 1) * st);
     const Model32 m = S.m;
     const float dt = S.dt;
     const int row = i - 1;
@@ -500,6 +500,350 @@ __global__ __launch_bounds__(NT, WPE) void k_fused_adj(const FusedSlot *__restri
     }
 }
 
+// ------------------------------------------------------------------------------ adjoint, two steps
+// Two adjoint steps per launch (MODE_ADJ2: i and then i - 1, both inside one segment) on a 64 x TY
+// tile with an 8-cell halo, for launches too small to fill the GPU: stages 1..4 (step i) run on the
+// cells within distance 7, 6, 5, 4 of the tile and stages 5..8 (step i - 1) within 3, 2, 1, 0, so the
+// intermediate adjoint state z_{i-1} never leaves LDS.  J(y_{i-1}) of step i's stage 4 is the
+// Jacobian of step i - 1's stage 1 and stays in registers.  Every cell's arithmetic is the one-step
+// kernel's, expression for expression, and each step writes its z.gh partial per 64 x 8 block, so
+// results equal two one-step launches bit for bit.
+template <int TY, int NT, int WPE>
+__global__ __launch_bounds__(NT, WPE) void k_fused_adj2(const FusedSlot *__restrict__ slots, Cursors cur, LaunchMap map)
+{
+    using G = Tile<TY, NT, 2>;
+    constexpr int RX = G::RXP, RY = G::RY, NS = G::NSLOT, TS = G::TS;
+    // z_i -> sA; y(t_{i-1} + dt/3) -> sB; Z_1 -> sX; Z_2 -> sA; Z_3 -> sX; z_{i-1} -> sA;
+    // y(t_{i-2} + dt/3) -> sB; Z_1' -> sX; Z_2' -> sA; Z_3' -> sX (one barrier per stage)
+    __shared__ float sA[3][RY][RX];
+    __shared__ float sB[3][RY][RX];
+    __shared__ float sX[3][RY][RX];
+    __shared__ uint16_t sC[RY][RX];
+    __shared__ double sP[TY];
+
+    const TileRef t = launch_tile<TY>(slots, map);
+    const SlotCursor sc = cur.c[t.slot];
+    if (sc.mode != MODE_ADJ2) return;   // uniform per block
+    const int i = sc.step, seg_a = sc.seg_a;   // steps i and i - 1 > seg_a
+    const FusedSlot &S = slots[t.slot];
+    const int tid = threadIdx.x;
+    const size_t st = (size_t)3 * S.n;
+    const int segi = seg_a / S.K;
+    const int b_end = (seg_a + S.K < S.nt) ? seg_a + S.K : S.nt;
+    const int buf = seg_buf(S, segi);
+    const float *kbuf = S.kseg + (size_t)buf * S.K * st;
+    const float *segb = S.seg + (size_t)buf * (S.K + 1) * st;
+    // y_{i-2}, y_{i-1}, y_i and their slopes (y_{i-1} is inside the segment: i - 1 > seg_a)
+    const auto *yA2 = gmem((i - 2 == seg_a) ? S.ck + (size_t)segi * st : segb + (size_t)(i - 2 - seg_a) * st);
+    const auto *yA = gmem(segb + (size_t)(i - 1 - seg_a) * st);
+    const auto *yB = gmem((i == b_end) ? S.ck + (size_t)(segi + 1) * st : segb + (size_t)(i - seg_a) * st);
+    const auto *kA2 = gmem(kbuf + (size_t)(i - 2 - seg_a) * st);
+    const auto *kA = gmem(kbuf + (size_t)(i - 1 - seg_a) * st);
+    const float *knext = (b_end == S.nt) ? S.kT
+                         : (segi + 1 >= S.keep0 ? S.kseg + (size_t)seg_buf(S, segi + 1) * S.K * st
+                                                : S.kfirst + (size_t)((segi + 1) % 3) * st);
+    const auto *kB = gmem((i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext);
+    const float *zin = S.zb + (size_t)(sc.zpar & 1) * st;
+    auto *zout = gmem(S.zb + (size_t)((sc.zpar & 1) ^ 1) * st);
+    const Model32 m = S.m;
+    const float dt = S.dt;
+    // step i: interval (i-1, i); step i - 1: interval (i-2, i-1) (the stored end slope f(y_{i-1})
+    // used the forward field of step i - 1)
+    const float E3a = (float)S.Ea[3 * (i - 1) + 2];
+    const float dEBa = (i < S.nt) ? (float)(S.Ea[3 * (i - 1) + 1] - S.Ef[3 * i]) : 0.0f;
+    const float E3b = (float)S.Ea[3 * (i - 2) + 2];
+    const float dEBb = (float)(S.Ea[3 * (i - 2) + 1] - S.Ef[3 * (i - 1)]);
+
+    int ly[NS], lx[NS], ls[NS];
+    uint32_t q[NS];
+    bool own[NS], act[NS];
+    uint32_t cc[NS];
+    float pa[NS][3], pb[NS][3], pka[NS][3], pkb[NS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        own[sl] = slot_cell<G>(sl, tid, ly[sl], lx[sl], ls[sl]);
+        if (!own[sl]) { ly[sl] = G::H; lx[sl] = G::H; }
+        const int kk = (t.y0 + ly[sl] - G::H) * S.pitch + (t.x0 + lx[sl] - G::H);
+        q[sl] = (uint32_t)min(max(kk, 0), S.n - 1);
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            pa[sl][v] = yA[v * S.n + q[sl]];
+            pb[sl][v] = yB[v * S.n + q[sl]];
+            pka[sl][v] = kA[v * S.n + q[sl]];
+            pkb[sl][v] = kB[v * S.n + q[sl]];
+        }
+    }
+
+    load_region<G>(S, zin, t.x0, t.y0, tid, sA, sC);
+    __syncthreads();
+
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        cc[sl] = sC[ly[sl]][lx[sl]];
+        act[sl] = own[sl] && (cc[sl] & C_ACTIVE);
+    }
+
+    // ---- step i
+    // phase 1a: y(t_{i-1}+dt/3) and the partial midpoint value from the stored end slopes
+    float y4[NS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        float y3[3] = {0.0f, 0.0f, 0.0f};
+        if (act[sl]) {
+            const float gh = m.ghc * (float)code_gh(cc[sl]);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float a = pa[sl][v], b = pb[sl][v];
+                float fb = pkb[sl][v];
+                if (v == 0) fb += dEBa * gh;
+                float m0 = pka[sl][v] * dt, m1 = fb * dt;
+                y3[v] = h3<float>(a, b, m0, m1);
+                y4[sl][v] = h4p<float>(a, b, m0, m1);
+            }
+        }
+        if (own[sl])
+#pragma unroll
+            for (int v = 0; v < 3; v++) sB[v][ly[sl]][lx[sl]] = y3[v];
+    }
+
+    // phase 1b: stage 1 with J(y_i)
+    float z[NS][3], dz[NS][3];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        float lt[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        z[sl][0] = z[sl][1] = z[sl][2] = 0.0f;
+        if (slot_in_stage<G>(sl, 4)) dz[sl][0] = dz[sl][1] = dz[sl][2] = 0.0f;
+        if (!own[sl]) continue;
+        lapT_lds<RY, RX>(m, sA, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, z[sl]);
+        if (act[sl]) {
+            Gates32 g = gates32(m, pb[sl][0]);
+            Jac32 J = jac32(m, g, pb[sl][0], pb[sl][1], pb[sl][2]);
+            jtz(J, z[sl], o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
+                if (slot_in_stage<G>(sl, 4)) dz[sl][v] = (1.0f / 6.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 2: midpoint state yh, J(yh), stage 2
+    Jac32 Jh[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 2) || ls[sl] < 2) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sX, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            float f3[3], yc[3], yh[3];
+            rhs_lds<RY, RX>(m, sB, lap_weights(m, cc[sl]), ly[sl], lx[sl], E3a, f3, yc);
+#pragma unroll
+            for (int v = 0; v < 3; v++) yh[v] = y4[sl][v] + (float)(27.0 / 64.0) * (f3[v] * dt);
+            Gates32 g = gates32(m, yh[0]);
+            Jh[sl] = jac32(m, g, yh[0], yh[1], yh[2]);
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
+                if (slot_in_stage<G>(sl, 4)) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 3: stage 3 with J(yh)
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 3) || ls[sl] < 3) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sA, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + dt * fz;
+                if (slot_in_stage<G>(sl, 4)) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 4: stage 4 with J(y_{i-1}) (kept for step i - 1's stage 1); z_{i-1} into LDS and, for the
+    // tile's rows, the z.gh sums of step i (row i - 1 of the partials)
+    Jac32 Jm[NS];
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 4) || ls[sl] < 4) continue;
+        float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sX, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            float ya[3];   // y_{i-1}, again from memory (fewer registers held across the stages)
+#pragma unroll
+            for (int v = 0; v < 3; v++) ya[v] = yA[v * S.n + q[sl]];
+            Gates32 g = gates32(m, ya[0]);
+            Jm[sl] = jac32(m, g, ya[0], ya[1], ya[2]);
+            jtz(Jm[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) zn[v] = z[sl][v] + (dz[sl][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) {
+            sA[v][ly[sl]][lx[sl]] = zn[v];
+            z[sl][v] = zn[v];
+        }
+        if (sl < TS) {
+            const bool in = t.y0 + ly[sl] - G::H < S.rows;
+            const double rs = wave_sum64(zgh_term(m, act[sl] && in, cc[sl], zn[0]));
+            if ((tid & 63) == 0) sP[ly[sl] - G::H] = rs;
+        }
+    }
+    __syncthreads();
+    static_assert(TY % PART_ROWS == 0, "tiles hold whole partial blocks");
+    if (tid < TY / PART_ROWS) {
+        const int brow = t.y0 / PART_ROWS + tid;
+        if (brow * PART_ROWS < S.rows) {
+            double s = 0.0;
+            for (int j = 0; j < PART_ROWS; j++) s += sP[tid * PART_ROWS + j];
+            gmem(S.partial)[(size_t)(i - 1) * S.ntiles + (size_t)brow * (S.pitch / FUSED_TX) + t.x0 / FUSED_TX] = s;
+        }
+    }
+
+    // ---- step i - 1
+    // phase 5a: y(t_{i-2}+dt/3) and the partial midpoint value, on the cells of stages 5 and 6
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 5)) continue;
+        float y3[3] = {0.0f, 0.0f, 0.0f};
+        if (act[sl] && ls[sl] >= 5) {
+            const float gh = m.ghc * (float)code_gh(cc[sl]);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float a = yA2[v * S.n + q[sl]], b = yA[v * S.n + q[sl]];
+                float fb = kA[v * S.n + q[sl]];
+                if (v == 0) fb += dEBb * gh;
+                float m0 = kA2[v * S.n + q[sl]] * dt, m1 = fb * dt;
+                y3[v] = h3<float>(a, b, m0, m1);
+                y4[sl][v] = h4p<float>(a, b, m0, m1);
+            }
+        }
+        if (own[sl] && ls[sl] >= 5)
+#pragma unroll
+            for (int v = 0; v < 3; v++) sB[v][ly[sl]][lx[sl]] = y3[v];
+    }
+
+    // phase 5b: stage 5 with J(y_{i-1})
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 5) || ls[sl] < 5) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        if (sl < TS) dz[sl][0] = dz[sl][1] = dz[sl][2] = 0.0f;
+        lapT_lds<RY, RX>(m, sA, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            jtz(Jm[sl], z[sl], o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
+                if (sl < TS) dz[sl][v] = (1.0f / 6.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 6: midpoint state, J, stage 6
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 6) || ls[sl] < 6) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sX, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            float f3[3], yc[3], yh[3];
+            rhs_lds<RY, RX>(m, sB, lap_weights(m, cc[sl]), ly[sl], lx[sl], E3b, f3, yc);
+#pragma unroll
+            for (int v = 0; v < 3; v++) yh[v] = y4[sl][v] + (float)(27.0 / 64.0) * (f3[v] * dt);
+            Gates32 g = gates32(m, yh[0]);
+            Jh[sl] = jac32(m, g, yh[0], yh[1], yh[2]);
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + 0.5f * dt * fz;
+                if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sA[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 7: stage 7
+#pragma unroll
+    for (int sl = 0; sl < NS; sl++) {
+        if (!slot_in_stage<G>(sl, 7) || ls[sl] < 7) continue;
+        float lt[3], zc[3], o[3], Zk[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sA, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            jtz(Jh[sl], zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) {
+                float fz = lt[v] + o[v];
+                Zk[v] = z[sl][v] + dt * fz;
+                if (sl < TS) dz[sl][v] += (1.0f / 3.0f) * dt * fz;
+            }
+        }
+#pragma unroll
+        for (int v = 0; v < 3; v++) sX[v][ly[sl]][lx[sl]] = Zk[v];
+    }
+    __syncthreads();
+
+    // phase 8: stage 8 on the tile with J(y_{i-2}); z_{i-2} and the z.gh partial of step i - 1
+#pragma unroll
+    for (int sl = 0; sl < TS; sl++) {
+        float lt[3], zc[3], o[3], zn[3] = {0.0f, 0.0f, 0.0f};
+        lapT_lds<RY, RX>(m, sX, lapT_weights(cc[sl]), ly[sl], lx[sl], lt, zc);
+        if (act[sl]) {
+            float ya[3];   // y_{i-2}
+#pragma unroll
+            for (int v = 0; v < 3; v++) ya[v] = yA2[v * S.n + q[sl]];
+            Gates32 g = gates32(m, ya[0]);
+            Jac32 J = jac32(m, g, ya[0], ya[1], ya[2]);
+            jtz(J, zc, o);
+#pragma unroll
+            for (int v = 0; v < 3; v++) zn[v] = z[sl][v] + (dz[sl][v] + (1.0f / 6.0f) * dt * (lt[v] + o[v]));
+        }
+        const int gy = t.y0 + ly[sl] - G::H;
+        const bool in = gy < S.rows;
+        if (in) {
+            const uint32_t kk = (uint32_t)(gy * S.pitch + t.x0 + lx[sl] - G::H);
+#pragma unroll
+            for (int v = 0; v < 3; v++) zout[v * S.n + kk] = zn[v];
+        }
+        const double rs = wave_sum64(zgh_term(m, act[sl] && in, cc[sl], zn[0]));
+        if ((tid & 63) == 0) sP[ly[sl] - G::H] = rs;
+    }
+    __syncthreads();
+    if (tid < TY / PART_ROWS) {
+        const int brow = t.y0 / PART_ROWS + tid;
+        if (brow * PART_ROWS < S.rows) {
+            double s = 0.0;
+            for (int j = 0; j < PART_ROWS; j++) s += sP[tid * PART_ROWS + j];
+            gmem(S.partial)[(size_t)(i - 2) * S.ntiles + (size_t)brow * (S.pitch / FUSED_TX) + t.x0 / FUSED_TX] = s;
+        }
+    }
+}
+
 // ------------------------------------------------------------------------------ launchers
 // Tile configurations: id -> (rows, threads).  1: 64x16 / 256, 2: 64x32 / 256, 3: 64x32 / 512,
 // 4: 64x16 / 512, 5: 64x32 / 1024, 6: 64x8 / 512, 7: 64x8 / 256.  The occupancy requests (third
@@ -531,6 +875,15 @@ void fused_forward2_launch(const FusedSlot *slots_dev, const LaunchMap &map, con
     const dim3 grid(launch_map_tiles(map));
     if (grid.x == 0) return;
     hipLaunchKernelGGL((k_fused_fwd<8, 512, 1, 2>), grid, dim3(512), 0, stream, slots_dev, cur, map);
+}
+
+// Two adjoint steps per launch (MODE_ADJ2) on 64x8 tiles of 512 threads with an 8-cell halo: for
+// launches too small to fill the GPU, as the forward sweep's two-step launches.
+void fused_adjoint2_launch(const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur, hipStream_t stream)
+{
+    const dim3 grid(launch_map_tiles(map));
+    if (grid.x == 0) return;
+    hipLaunchKernelGGL((k_fused_adj2<8, 512, 4>), grid, dim3(512), 0, stream, slots_dev, cur, map);
 }
 
 void fused_adjoint_launch(int cfg, const FusedSlot *slots_dev, const LaunchMap &map, const Cursors &cur,

@@ -687,8 +687,8 @@ __global__ __launch_bounds__(64 * NW, 4) void k_roll_adj(const FusedSlot *__rest
                          : (segi + 1 >= S.keep0 ? S.kseg + (size_t)seg_buf(S, segi + 1) * S.K * st
                                                 : S.kfirst + (size_t)((segi + 1) % 3) * st);
     A.kB = buf_rsrc((i < b_end) ? kbuf + (size_t)(i - seg_a) * st : knext);
-    A.zin = buf_rsrc(S.zb + (size_t)((S.nt - i) & 1) * st);
-    A.zout = buf_rsrc(S.zb + (size_t)((S.nt - i + 1) & 1) * st);
+    A.zin = buf_rsrc(S.zb + (size_t)(sc.zpar & 1) * st);
+    A.zout = buf_rsrc(S.zb + (size_t)((sc.zpar & 1) ^ 1) * st);
     A.m = S.m;
     A.dt = S.dt;
     A.row_i = i - 1;

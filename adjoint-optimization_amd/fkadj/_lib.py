@@ -43,6 +43,7 @@ class Problem(C.Structure):
         ("precision", C.c_int), ("ckpt", C.c_int), ("fwd_tile", C.c_int), ("adj_tile", C.c_int),
         ("lazy_workspace", C.c_int),
         ("keep_bytes", C.c_int64),
+        ("fwd_depth", C.c_int), ("adj_depth", C.c_int), ("roll_rows", C.c_int),
     ]
 
 

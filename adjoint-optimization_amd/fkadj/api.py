@@ -89,11 +89,15 @@ class Instance:
     """A device-resident gradient problem (one grid, one objective, one resident state).
 
     keep_bytes (FP32): device bytes for keeping the last checkpoint segments of the forward
-    trajectory, which the backward pass then does not recompute (same results, fewer launches)."""
+    trajectory, which the backward pass then does not recompute (same results, fewer launches).
+    fwd_depth, adj_depth (FP32): time steps per launch of the tiled kernels (1 or 2; 0 = the
+    library's rule); roll_rows (FP32): strip rows of the row-rolling kernels (a multiple of 8; 0 =
+    the library setting).  None of them changes a result."""
 
     def __init__(self, active_index, nrows, ncols, *, dt, nt, model=None, ghost_lr=None, pulse_mode=L.PULSE_KNOTS,
                  n_amp=0, alpha=0.5, gamma=(1.0, 0.2, 0.2), S=None, s_diag=0.0, precision=L.F32, ckpt=50,
-                 fwd_tile=0, adj_tile=0, lazy_workspace=False, keep_bytes=0):
+                 fwd_tile=0, adj_tile=0, lazy_workspace=False, keep_bytes=0, fwd_depth=0, adj_depth=0,
+                 roll_rows=0):
         self._keep = []
         ai = np.ascontiguousarray(np.ravel(active_index), dtype=np.int32)
         p = L.Problem()
@@ -120,6 +124,7 @@ class Instance:
         p.precision, p.ckpt, p.fwd_tile, p.adj_tile = int(precision), int(ckpt), int(fwd_tile), int(adj_tile)
         p.lazy_workspace = 1 if lazy_workspace else 0
         p.keep_bytes = int(keep_bytes)
+        p.fwd_depth, p.adj_depth, p.roll_rows = int(fwd_depth), int(adj_depth), int(roll_rows)
         self.problem = p
         self.nrows, self.ncols, self.nt, self.dt = p.nrows, p.ncols, p.nt, p.dt
         self.precision = p.precision

@@ -1,24 +1,28 @@
-"""Tile autotuner and profiling driver for the fused FP32 kernels.
+"""Launch-shape autotuner for the FP32 path: tile shape, time-blocking depth and strip height.
 
-The fused kernels come in tile configurations (64 x TY cells per workgroup of NT threads):
-1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024, 6 = 64x8/512,
-7 = 64x8/256.  The forward tile never
-changes a result; the adjoint tile regroups the per-tile z.gh partial sums (rounding-level changes),
-so it is fixed per grid size.  Time blocking: at these launch sizes one step per launch (the kernels
-are bound by vector-ALU issue, not by HBM, so a deeper halo only adds recomputation); the library
-takes two forward-sweep steps per launch on its own for launches under 1M cells on the built-in
-forward tile, which a tuned forward tile turns off.
+None of these choices changes a result (the tests check that bit for bit); they only change how
+the work is cut into launches.  Per grid size N the tuner times whole gradient evaluations of a
+synthetic instance (the benchmark's geometry: 80 disc holes per 512^2, h = 0.025 cm, dt = 0.02 ms,
+200 amplitudes, half-plane excitation, T = 20 ms by default) under each candidate and stores the
+fastest in tuned_tiles.json, at two launch scales:
 
-    python -m fkadj.autotune --grid 512 --batch 16            # sweep one size, print a table
-    python -m fkadj.autotune --sizes 256 384 512 768 1024 --write
-                                    # sweep each grid size at about 4M cells per launch and store
-                                    # the best pair per size in tuned_tiles.json
-    python -m fkadj.autotune --grid 512 --batch 16 --once 3 4 # one evaluation (profiler target)
+* "single": one instance per launch, on the tiled kernels.  Candidates: forward tile and depth
+  (64x8/512 cells/threads with one or two steps per launch, 64x16/512 with one), then adjoint tile
+  and depth (64x8/512 with one or two steps per launch, 64x16/512 with one), the adjoint swept with
+  the best forward choice.
+* "batch": instances batched to about 4M cells per launch.  Candidates: the row-rolling kernels
+  with strips of 8, 16 or 32 rows (pitches 256, 384, 512, 768, 1024), and the tiled kernels
+  (64x16/512) for every pitch.
 
-The synthetic batch uses the benchmark's geometry (80 disc holes per 512^2, h = 0.025 cm,
-dt = 0.02 ms, 200 amplitudes) with a cheap initial state (half-plane excitation) and T = 20 ms.
-Instances pick their tiles by grid size with lookup(N); the forward tile changes no result and
-the adjoint tile only the grouping of the FP64 partial sums.
+Tile ids: 1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024,
+6 = 64x8/512, 7 = 64x8/256.
+
+    python -m fkadj.autotune --sizes 128 256 512 --write      # sweep, store in tuned_tiles.json
+    python -m fkadj.autotune --grid 512 --scale single         # sweep one size, print only
+    python -m fkadj.autotune --grid 512 --batch 16 --once 0 4  # one evaluation (profiler target)
+
+instance_kwargs(N, batched) returns the stored choice as Instance / make_instance keyword
+arguments (the library's built-in rules where nothing is stored).
 """
 from __future__ import annotations
 
@@ -34,16 +38,22 @@ from . import workloads as W
 from .api import batch_grad
 
 TABLE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "tuned_tiles.json")
-FWD_CFGS = (1, 2, 3, 4, 5, 6, 7)
-ADJ_CFGS = (1, 3, 4, 5, 6, 7)
+ROLL_PITCHES = (256, 384, 512, 768, 1024)
+
+# single-instance candidates: (fwd_tile, fwd_depth) and (adj_tile, adj_depth)
+FWD_SINGLE = ((6, 2), (6, 1), (4, 1))
+ADJ_SINGLE = ((6, 1), (4, 1), (6, 2))
+# batched candidates: strip rows of the rolling kernels (0: tiled kernels only)
+ROLL_ROWS = (8, 16, 32)
 
 
-def synthetic_batch(N: int, B: int, T: float, fwd_tile: int, adj_tile: int, holes: int | None = None):
+def synthetic_batch(N: int, B: int, T: float, holes: int | None = None, **kw):
+    """B instances of an N^2 grid (seeded disc holes, half-plane excitation), Instance keywords kw."""
     insts = []
     holes = holes if holes is not None else max(1, round(80 * N * N / 512 ** 2))
     for s in range(B):
         m = W.holes_mask(N, holes, s)
-        inst = W.make_instance(m, T, 200, precision=L.F32, ckpt=50, fwd_tile=fwd_tile, adj_tile=adj_tile)
+        inst = W.make_instance(m, T, 200, precision=L.F32, ckpt=50, **kw)
         inst.set_state(W.rest_state(int(m.sum())))
         s1 = np.zeros(m.shape, bool)
         s1[:, : N // 2] = True
@@ -52,24 +62,64 @@ def synthetic_batch(N: int, B: int, T: float, fwd_tile: int, adj_tile: int, hole
     return insts, [W.amplitudes(200, s) for s in range(B)]
 
 
-def time_pair(N: int, B: int, T: float, ft: int, at: int, reps: int = 1) -> dict:
-    insts, A = synthetic_batch(N, B, T, ft, at)
+def time_config(N: int, B: int, T: float, reps: int = 2, roll_mode: int = 1, **kw) -> dict:
+    """Best of `reps` wall-clock batched evaluations (after one warm-up) under the given Instance
+    keywords; the rolling kernels are allowed (roll_mode 1, the library default) or not (0)."""
+    prev = L.set_rolling(roll_mode, 16)
+    insts, A = synthetic_batch(N, B, T, **kw)
     try:
         batch_grad(insts, A)
-        L.set_profiling(True)
         best = None
         for _ in range(reps):
             t0 = time.perf_counter()
-            batch_grad(insts, A)
+            res = batch_grad(insts, A)
             el = time.perf_counter() - t0
-            tm = L.last_timing()
-            if best is None or el < best["eval_s"]:
-                best = {"eval_s": el, "fwd_us": 1e3 * tm["fwd_kernel_ms"], "adj_us": 1e3 * tm["adj_kernel_ms"]}
-        L.set_profiling(False)
-        return best
+            best = el if best is None else min(best, el)
+        return {"ms": 1e3 * best, "grad0": res[0][0]}
     finally:
         for i in insts:
             i.close()
+        L.set_rolling(prev, 16)
+
+
+def sweep_single(N: int, T: float) -> dict:
+    """Forward (tile, depth) first with the built-in adjoint, then adjoint (tile, depth)."""
+    fw = {}
+    for ft, fd in FWD_SINGLE:
+        fw[ft, fd] = time_config(N, 1, T, fwd_tile=ft, fwd_depth=fd)
+        print(f"{N}^2 single fwd_tile={ft} fwd_depth={fd}: {fw[ft, fd]['ms']:.1f} ms/eval", flush=True)
+    bf = min(fw, key=lambda c: fw[c]["ms"])
+    ad = {}
+    for at, adp in ADJ_SINGLE:
+        ad[at, adp] = time_config(N, 1, T, fwd_tile=bf[0], fwd_depth=bf[1], adj_tile=at, adj_depth=adp)
+        print(f"{N}^2 single adj_tile={at} adj_depth={adp}: {ad[at, adp]['ms']:.1f} ms/eval", flush=True)
+    ba = min(ad, key=lambda c: ad[c]["ms"])
+    ref = fw[FWD_SINGLE[0]]["grad0"]
+    same = all(np.array_equal(r["grad0"], ref) for r in list(fw.values()) + list(ad.values()))
+    print(f"{N}^2 single best: fwd {bf} adj {ba}; all candidates give the same gradient: {same}", flush=True)
+    return {"fwd_tile": bf[0], "fwd_depth": bf[1], "adj_tile": ba[0], "adj_depth": ba[1], "ms": ad[ba]["ms"],
+            "candidates_ms": {**{f"fwd {k}": v["ms"] for k, v in fw.items()},
+                              **{f"adj {k}": v["ms"] for k, v in ad.items()}},
+            "same_gradient": same}
+
+
+def sweep_batch(N: int, T: float, cells: int = 1 << 22) -> dict:
+    """Rolling strip rows (pitches the rolling kernels take) against the tiled kernels."""
+    B = min(64, max(1, round(cells / (N * N))))
+    pitch = (N + 63) // 64 * 64
+    res = {0: time_config(N, B, T, roll_mode=0, fwd_tile=4, adj_tile=4)}
+    print(f"{N}^2 x{B} tiled 64x16/512: {res[0]['ms']:.1f} ms/eval", flush=True)
+    if pitch in ROLL_PITCHES:
+        for hs in ROLL_ROWS:
+            res[hs] = time_config(N, B, T, roll_mode=2, fwd_tile=4, adj_tile=4, roll_rows=hs)
+            print(f"{N}^2 x{B} rolling strips of {hs} rows: {res[hs]['ms']:.1f} ms/eval", flush=True)
+    best = min(res, key=lambda c: res[c]["ms"])
+    same = all(np.array_equal(r["grad0"], res[0]["grad0"]) for r in res.values())
+    print(f"{N}^2 x{B} best: {'tiled' if best == 0 else f'rolling, {best}-row strips'}; same gradient: {same}",
+          flush=True)
+    return {"batch": B, "roll_rows": best, "fwd_tile": 4, "adj_tile": 4, "ms": res[best]["ms"],
+            "candidates_ms": {("tiled" if k == 0 else f"roll {k}"): v["ms"] for k, v in res.items()},
+            "same_gradient": same}
 
 
 def load_table(path: str = TABLE) -> dict:
@@ -79,54 +129,45 @@ def load_table(path: str = TABLE) -> dict:
     return {}
 
 
-def lookup(N: int) -> tuple[int, int]:
-    """Tuned (fwd_tile, adj_tile) for an N^2 grid: the entry of the nearest tuned size, or (0, 0)
-    (the library's built-in choice) when nothing is tuned."""
-    t = load_table()
+def instance_kwargs(N: int, batched: bool, path: str = TABLE) -> dict:
+    """Instance keywords of the tuned choice for an N^2 grid at the given launch scale (the entry of
+    the nearest tuned size), or {} (the library's rules) when nothing is tuned for that scale."""
+    t = {k: v for k, v in load_table(path).items() if ("batch" if batched else "single") in v}
     if not t:
-        return 0, 0
-    key = min(t, key=lambda k: abs(int(k) - N))
-    return int(t[key]["fwd_tile"]), int(t[key]["adj_tile"])
-
-
-def sweep(N: int, B: int, T: float) -> dict:
-    """Forward tile first (adjoint at the built-in choice), then the adjoint tile."""
-    res_f = {ft: time_pair(N, B, T, ft, 0) for ft in FWD_CFGS}
-    for ft, r in res_f.items():
-        print(f"{N}^2 x{B} fwd_tile={ft}: forward {r['fwd_us']:.1f} us/launch", flush=True)
-    best_f = min(res_f, key=lambda c: res_f[c]["fwd_us"])
-    res_a = {at: time_pair(N, B, T, best_f, at) for at in ADJ_CFGS}
-    for at, r in res_a.items():
-        print(f"{N}^2 x{B} adj_tile={at}: adjoint {r['adj_us']:.1f} us/launch, eval {1e3 * r['eval_s']:.1f} ms",
-              flush=True)
-    best_a = min(res_a, key=lambda c: res_a[c]["adj_us"])
-    print(f"{N}^2 x{B} best: fwd_tile={best_f} adj_tile={best_a}", flush=True)
-    return {"fwd_tile": best_f, "adj_tile": best_a, "batch": B, "fwd_us": res_f[best_f]["fwd_us"],
-            "adj_us": res_a[best_a]["adj_us"]}
+        return {}
+    e = t[min(t, key=lambda k: abs(int(k) - N))]["batch" if batched else "single"]
+    if batched:
+        if e["roll_rows"] > 0:
+            return {"roll_rows": int(e["roll_rows"])}
+        return {"fwd_tile": int(e["fwd_tile"]), "adj_tile": int(e["adj_tile"])}
+    return {k: int(e[k]) for k in ("fwd_tile", "fwd_depth", "adj_tile", "adj_depth")}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--grid", type=int, default=512)
     ap.add_argument("--batch", type=int, default=16)
-    ap.add_argument("--sizes", type=int, nargs="+", help="sweep these grid sizes at --cells cells per launch")
-    ap.add_argument("--cells", type=int, default=1 << 22)
+    ap.add_argument("--sizes", type=int, nargs="+", help="sweep these grid sizes")
+    ap.add_argument("--scale", choices=("single", "batch", "both"), default="both")
     ap.add_argument("--T", type=float, default=20.0)
-    ap.add_argument("--once", type=int, nargs=2, metavar=("FWD", "ADJ"))
+    ap.add_argument("--once", type=int, nargs=2, metavar=("FWD", "ADJ"),
+                    help="one batched evaluation with these forward and adjoint tiles")
     ap.add_argument("--write", action="store_true")
     ap.add_argument("--table", default=TABLE, help="table file read and written by --write")
     args = ap.parse_args()
     if args.once:
-        insts, A = synthetic_batch(args.grid, args.batch, args.T, *args.once)
+        insts, A = synthetic_batch(args.grid, args.batch, args.T, fwd_tile=args.once[0], adj_tile=args.once[1])
         t0 = time.perf_counter()
         batch_grad(insts, A)
         print(f"one evaluation: {time.perf_counter() - t0:.3f} s", flush=True)
         return
-    sizes = args.sizes or [args.grid]
     t = load_table(args.table)
-    for N in sizes:
-        B = args.batch if not args.sizes else min(64, max(1, round(args.cells / (N * N))))   # <= 64 per launch
-        t[str(N)] = sweep(N, B, args.T)
+    for N in args.sizes or [args.grid]:
+        e = t.setdefault(str(N), {})
+        if args.scale in ("single", "both"):
+            e["single"] = sweep_single(N, args.T)
+        if args.scale in ("batch", "both"):
+            e["batch"] = sweep_batch(N, args.T)
     if args.write:
         with open(args.table, "w") as f:
             json.dump(t, f, indent=1, sort_keys=True)

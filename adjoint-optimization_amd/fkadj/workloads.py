@@ -55,13 +55,14 @@ def rest_state(n_act: int) -> np.ndarray:
 
 
 def make_instance(mask: np.ndarray, T: float, K: int, precision=L.F32, ckpt=50, fwd_tile=0, adj_tile=0,
-                  keep_bytes=0, lazy_workspace=False) -> Instance:
+                  keep_bytes=0, lazy_workspace=False, fwd_depth=0, adj_depth=0, roll_rows=0) -> Instance:
     nt = int(round(T / DT_MS))
     n_act = int(mask.sum())
     return Instance(G.active_index_from_mask(mask), mask.shape[0], mask.shape[1], dt=DT_MS, nt=nt, model=model(),
                     pulse_mode=L.PULSE_PIECEWISE, n_amp=K, alpha=2 * LAMBDA, gamma=(1.0, 0.0, 0.0),
                     s_diag=2.0 / n_act, precision=precision, ckpt=ckpt, fwd_tile=fwd_tile,
-                    adj_tile=adj_tile, keep_bytes=keep_bytes, lazy_workspace=lazy_workspace)
+                    adj_tile=adj_tile, keep_bytes=keep_bytes, lazy_workspace=lazy_workspace, fwd_depth=fwd_depth,
+                    adj_depth=adj_depth, roll_rows=roll_rows)
 
 
 def s2_region(nrows: int, ncols: int, back_col: int, bands: int) -> np.ndarray:

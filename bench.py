@@ -292,15 +292,14 @@ def run_c5(args):
     log(f"[c5] four-spiral state ready in {time.perf_counter() - t0:.1f} s ({ps} phase singularities)")
     A = W.amplitudes(K_amp, seed)
     from fkadj import autotune
-    _, at = autotune.lookup(N)
-    ft = 0
+    tk = autotune.instance_kwargs(N, batched=False)   # tiles and depths tuned for a lone instance
     # spare HBM (all but the instance's own buffers and 16 GB; the FP64 instance comes after this
     # one is closed) keeps the last trajectory segments
     free0, _ = fkadj.mem_info()
-    spare = 0 if args.no_keep else free0 - footprint(N, N, nt, ckpt, fkadj.F32, at) - (16 << 30)
+    spare = 0 if args.no_keep else free0 - footprint(N, N, nt, ckpt, fkadj.F32, tk.get("adj_tile", 0)) - (16 << 30)
     kept, (keep,) = kept_budgets([(N, N)], nt, ckpt, spare)
     log(f"[c5] keeping {kept} of {(nt + ckpt - 1) // ckpt} segments ({keep / 1e9:.0f} GB)")
-    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=ft, adj_tile=at, keep_bytes=keep)
+    inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=ckpt, keep_bytes=keep, **tk)
     inst.set_state(y0)
     for _ in range(args.warmup):
         inst.grad(A)
@@ -336,7 +335,7 @@ def run_c5(args):
         Ts, Ks = 100.0, 100
         gs = {}
         for prec in (fkadj.F32, fkadj.F64):
-            ish = W.make_instance(m, Ts, Ks, precision=prec, ckpt=ckpt, fwd_tile=ft, adj_tile=at)
+            ish = W.make_instance(m, Ts, Ks, precision=prec, ckpt=ckpt, **(tk if prec == fkadj.F32 else {}))
             ish.set_state(y0)
             gs[prec] = ish.grad(A[:Ks])[0]
             ish.close()
@@ -388,15 +387,15 @@ def run_c3(args):
     # after the batch's own buffers, the FP64 instance and 16 GB
     k0 = int(np.argmin(sizes))
     free0, _ = fkadj.mem_info()
-    own = sum(footprint(n, n, nt, 50, fkadj.F32, autotune.lookup(n)[1]) for n in sizes)
+    tuned = {n: autotune.instance_kwargs(n, batched=True) for n in set(sizes)}   # per grid size
+    own = sum(footprint(n, n, nt, 50, fkadj.F32, tuned[n].get("adj_tile", 0)) for n in sizes)
     spare = 0 if args.no_keep else free0 - own - footprint(sizes[k0], sizes[k0], nt, 50, fkadj.F64) - (16 << 30)
     kept, keeps = kept_budgets([(n, n) for n in sizes], nt, 50, spare)
     log(f"[c3] keeping {kept} of {(nt + 49) // 50} segments of every instance ({sum(keeps) / 1e9:.0f} GB)")
     insts = []
     for m, y, keep in zip(masks, states, keeps):
-        _, at = autotune.lookup(m.shape[0])        # adjoint tile tuned per grid size
-        ft = 0                                      # forward tile: built-in rule by launch size
-        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at, keep_bytes=keep)
+        # launch shapes tuned per grid size at batch scale (strip rows of the rolling kernels, or tiles)
+        inst = W.make_instance(m, T, K_amp, precision=fkadj.F32, ckpt=50, keep_bytes=keep, **tuned[m.shape[0]])
         inst.set_state(y)
         insts.append(inst)
     for w in range(args.warmup):
@@ -468,15 +467,12 @@ def run_c4(args):
     t0 = time.perf_counter()
     masks, states = prepare(sizes, seeds)
     log(f"[c4] {P} instance states ready in {time.perf_counter() - t0:.1f} s")
-    from fkadj import autotune
     insts = []
     for m, y, T in zip(masks, states, Ts):
-        _, at = autotune.lookup(m.shape[0])
-        ft = 0
         inst = fkadj.Instance(fkadj.geometry.active_index_from_mask(m), m.shape[0], m.shape[1], dt=W.DT_MS,
                               nt=int(round(T / W.DT_MS)), model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE,
                               n_amp=T, alpha=2 * W.LAMBDA, gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()),
-                              precision=fkadj.F32, ckpt=50, fwd_tile=ft, adj_tile=at, lazy_workspace=True)
+                              precision=fkadj.F32, ckpt=50, lazy_workspace=True)
         inst.set_state(y)
         insts.append(inst)
     arrivals = np.cumsum(rng.exponential(1.0 / rate, R))

@@ -122,6 +122,18 @@ typedef struct fkadj_problem {
                                          the segments before them.  Results are unchanged.  0: none
                                          (every segment is recomputed); FP64 and server requests keep
                                          none */
+    int fwd_depth;                    /* FP32 forward-sweep time steps per launch on the tiled kernels:
+                                         1 or 2 (two needs nt and ckpt even); 0 -> built-in (2 for
+                                         launches under 1M cells on the built-in forward tile) */
+    int adj_depth;                    /* FP32 adjoint time steps per launch on the tiled kernels: 1 or
+                                         2 (two steps of one segment per launch, 64x8 tiles); 0 ->
+                                         built-in (1).  The members of a batch group take 2 only if
+                                         all ask for it.  Neither depth changes a result */
+    int roll_rows;                    /* FP32: strip rows of the row-rolling kernels when this instance
+                                         runs on them (a multiple of 8, 8..256): one workgroup per
+                                         strip; the members of one pitch in a batch group use it when
+                                         all ask for the same.  0 -> the library setting
+                                         (fkadj_set_rolling).  Never changes a result */
 } fkadj_problem;
 
 typedef struct fkadj_instance fkadj_instance;

@@ -1,5 +1,6 @@
 """C-ABI checks that need no GPU: the library loads, exports every declared entry point, and
 argument / geometry validation returns error codes before any device work."""
+import json
 import os
 import re
 
@@ -174,16 +175,25 @@ def test_grid_too_large_for_32_bit_offsets():
     assert fkadj.lib().fkadj_create(C.byref(p), C.byref(h)) == L.ERR_ARG
 
 
-def test_tile_table_lookup(tmp_path, monkeypatch):
+def test_tuned_table_lookup(tmp_path, monkeypatch):
+    """The autotuner's table gives Instance keywords by the nearest tuned grid size and launch scale;
+    the library's rules (no keywords) where nothing is tuned for that scale."""
     from fkadj import autotune
     t = tmp_path / "t.json"
-    t.write_text('{"256": {"fwd_tile": 1, "adj_tile": 4}, "1024": {"fwd_tile": 3, "adj_tile": 1}}')
-    monkeypatch.setattr(autotune, "load_table", lambda path=None: autotune.json.loads(t.read_text()))
-    assert autotune.lookup(256) == (1, 4)
-    assert autotune.lookup(300) == (1, 4)
-    assert autotune.lookup(2048) == (3, 1)
+    t.write_text(json.dumps({
+        "256": {"single": {"fwd_tile": 6, "fwd_depth": 2, "adj_tile": 6, "adj_depth": 1, "ms": 1.0},
+                "batch": {"roll_rows": 16, "fwd_tile": 4, "adj_tile": 4, "batch": 64, "ms": 2.0}},
+        "1024": {"single": {"fwd_tile": 4, "fwd_depth": 1, "adj_tile": 4, "adj_depth": 1, "ms": 3.0},
+                 "batch": {"roll_rows": 0, "fwd_tile": 4, "adj_tile": 4, "batch": 4, "ms": 4.0}},
+        "2048": {"single": {"fwd_tile": 4, "fwd_depth": 1, "adj_tile": 6, "adj_depth": 2, "ms": 5.0}}}))
+    assert autotune.instance_kwargs(300, False, str(t)) == {"fwd_tile": 6, "fwd_depth": 2, "adj_tile": 6,
+                                                            "adj_depth": 1}
+    assert autotune.instance_kwargs(256, True, str(t)) == {"roll_rows": 16}
+    assert autotune.instance_kwargs(900, True, str(t)) == {"fwd_tile": 4, "adj_tile": 4}
+    assert autotune.instance_kwargs(4096, True, str(t)) == {"fwd_tile": 4, "adj_tile": 4}   # nearest batch entry
+    assert autotune.instance_kwargs(2048, False, str(t))["adj_depth"] == 2
     monkeypatch.setattr(autotune, "load_table", lambda path=None: {})
-    assert autotune.lookup(512) == (0, 0)
+    assert autotune.instance_kwargs(512, True) == {} and autotune.instance_kwargs(512, False) == {}
 
 
 def test_forward_ref_argument_errors_before_device_work():

@@ -117,7 +117,9 @@ def test_c3_full_size_plan_batch_and_alone():
     A = [W.amplitudes(K, s) for s in seeds]
     insts = []
     for m, y in zip(masks, states):
-        inst = W.make_instance(m, T, K, precision=fkadj.F32, ckpt=50, adj_tile=autotune.lookup(m.shape[0])[1])
+        # launch shapes tuned per grid size at batch scale (they change no result)
+        inst = W.make_instance(m, T, K, precision=fkadj.F32, ckpt=50,
+                               **autotune.instance_kwargs(m.shape[0], batched=True))
         inst.set_state(y)
         insts.append(inst)
     batched = fkadj.batch_grad(insts, A)
@@ -165,7 +167,7 @@ def test_c4_poisson_stream_at_full_size_equals_alone():
         inst = fkadj.Instance(G.active_index_from_mask(m), N, N, dt=W.DT_MS, nt=int(round(T / W.DT_MS)),
                               model=W.model(), pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=T, alpha=2 * W.LAMBDA,
                               gamma=(1, 0, 0), s_diag=2.0 / int(m.sum()), precision=fkadj.F32, ckpt=50,
-                              adj_tile=autotune.lookup(N)[1], lazy_workspace=True)
+                              lazy_workspace=True)
         inst.set_state(_front_state(m, 300 + k))
         insts.append(inst)
         Es.append(W.amplitudes(T, 1000 + k))

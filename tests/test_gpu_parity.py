@@ -562,6 +562,42 @@ def test_two_step_forward_launches_change_nothing():
         assert np.array_equal(a[0], b[0]) and a[1] == b[1]
 
 
+def test_two_step_adjoint_launches_change_nothing():
+    """adj_depth = 2 takes two adjoint steps per launch (64x8 tiles, an 8-cell halo, J(y_{i-1}) of the
+    first step kept for the second); adj_depth = 1 keeps one step per launch.  Both give the same
+    gradient and J bit for bit, alone and batched, for segments of even and odd length (nt = 63 with
+    K = 10 ends with a 3-step segment, K = 7 makes every segment odd), piecewise and knot-sampled
+    fields, and kept segments; adj_depth out of {0, 1, 2} is refused."""
+    m = fkadj.reference_model(h=0.025)
+    dt = 0.02
+    shapes = [(64, 64), (96, 128)]
+    for nt, K, mode in ((60, 10, fkadj.PULSE_PIECEWISE), (63, 10, fkadj.PULSE_KNOTS), (63, 7, fkadj.PULSE_KNOTS)):
+        _print_case("adjoint_depth2", sizes="(64,64),(96,128)", nt=nt, K=K, mode=mode)
+        res = {}
+        for ad in (1, 2):
+            insts, Es = [], []
+            for k, (H, W) in enumerate(shapes):
+                case = disc_case(H, W, 6 + 2 * k, 50 + k)
+                kw = dict(pulse_mode=mode, n_amp=3) if mode == fkadj.PULSE_PIECEWISE else {}
+                seg = (2 * K + 1) * 3 * 4 * H * ((W + 63) // 64 * 64)
+                inst = fkadj.Instance(case.ai, H, W, dt=dt, nt=nt, model=m, alpha=2e-3, gamma=(1, 0, 0),
+                                      s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=K, adj_depth=ad,
+                                      keep_bytes=2 * seg if k == 1 else 0, **kw)
+                inst.set_state(case.y0)
+                insts.append(inst)
+                Es.append(np.random.default_rng(k).uniform(0, 10, 3) if mode == fkadj.PULSE_PIECEWISE
+                          else _Et(nt, k))
+            res[ad] = [insts[0].grad(Es[0]), insts[1].grad(Es[1])] + fkadj.batch_grad(insts, Es)
+            for inst in insts:
+                inst.close()
+        for a, b in zip(res[1], res[2]):
+            assert np.array_equal(a[0], b[0]) and a[1] == b[1], (nt, K, mode)
+    case = disc_case(64, 64, 6, 50)
+    for bad in (-1, 3):
+        with pytest.raises(fkadj.FkadjError):
+            fkadj.Instance(case.ai, 64, 64, dt=dt, nt=20, model=m, precision=fkadj.F32, adj_depth=bad)
+
+
 def test_kept_segments_change_nothing():
     """Kept segments (keep_bytes) are filled by the forward sweep and not recomputed: every budget,
     from none to all segments, gives the same gradient and J bit for bit, alone and batched with
@@ -657,17 +693,18 @@ def test_batch_forward_equals_forward_alone():
 def test_stimulus_and_launch_accounting():
     """fkadj_stimulate sets u on the masked active cells only; with profiling on, the last timing
     reports the launches of the evaluation: nt forward-sweep steps (two per launch for small
-    groups on the built-in forward tile, one per launch with an explicit tile), nt recompute and nt
-    adjoint launches."""
+    groups on the built-in forward tile, one per launch with an explicit tile), nt recompute
+    launches and nt adjoint steps (one per launch, two with adj_depth = 2).  The
+    segments are long enough (K = 20) that each kind's launch timing gets a full sampling window."""
     _print_case("stimulus_timing", shape=(48, 80))
     H, W, nt, K = 48, 80, 40, 4
     case = disc_case(H, W, 4, 61)
     m = fkadj.reference_model(h=0.025)
     counts = {}
-    for ft in (0, 6):
+    for ft, ad in ((0, 0), (6, 0), (0, 2)):
         with fkadj.Instance(case.ai, H, W, dt=0.02, nt=nt, model=m, pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=K,
-                            alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=10,
-                            fwd_tile=ft) as inst:
+                            alpha=2e-3, gamma=(1, 0, 0), s_diag=2.0 / case.n_act, precision=fkadj.F32, ckpt=20,
+                            fwd_tile=ft, adj_depth=ad) as inst:
             # values exact in FP32, so the resident state reads back unchanged
             y = np.concatenate([np.zeros(case.n_act), np.ones(case.n_act), np.full(case.n_act, 0.75)])
             inst.set_state(y)
@@ -684,8 +721,9 @@ def test_stimulus_and_launch_accounting():
                 t = fkadj.last_timing()
             finally:
                 fkadj.set_profiling(False)
-            counts[ft] = (t["fwd_launches"], t["adj_launches"], t["cell_steps"])
+            counts[ft, ad] = (t["fwd_launches"], t["adj_launches"], t["cell_steps"])
             assert t["eval_ms"] > 0 and t["fwd_kernel_ms"] > 0 and t["adj_kernel_ms"] > 0
     pitch = (W + 63) // 64 * 64
-    assert counts[0] == (nt // 2 + nt, nt, H * pitch * nt)
-    assert counts[6] == (2 * nt, nt, H * pitch * nt)
+    assert counts[0, 0] == (nt // 2 + nt, nt, H * pitch * nt)
+    assert counts[6, 0] == (2 * nt, nt, H * pitch * nt)
+    assert counts[0, 2] == (nt // 2 + nt, nt // 2, H * pitch * nt)

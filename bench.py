@@ -37,8 +37,8 @@ HBM_PEAK_GBS = 8000.0
 # 2 097 152 cells per dispatch); tiled kernels (64 x 16 cells, 512 threads) on one 2048^2 instance
 # (4 194 304 cells per dispatch, the C5 launch shape).
 FAMILIES = {
-    "roll": {"profile": "profiles/roll_c2x16_counters.md", "bytes": {"adj": 111.4, "fwd": 40.0},
-             "valu": {"adj": 675.0, "fwd": 378.0},
+    "roll": {"profile": "profiles/roll_c2x16_counters.md", "bytes": {"adj": 89.3, "fwd": 32.8},
+             "valu": {"adj": 654.0, "fwd": 378.0},
              "names": {"adj": "k_roll_adj<NW> (adjoint step, row-rolling)",
                        "fwd": "k_roll_fwd<NW> (forward/recompute step, row-rolling)"}},
     "tiled": {"profile": "profiles/fused_c5_2048_counters.md", "bytes": {"adj": 78.8, "fwd": 33.2},
@@ -652,10 +652,12 @@ def main():
 
     # instance 0 alone (its kept segments too), then single-instance FP32 latency on an instance
     # that keeps no segments (a lone 512^2 instance overlaps its recompute with the adjoint on a
-    # second stream, so keeping gains it nothing): all results are the same bit for bit, also the
-    # strict schedule's
+    # second stream, so keeping gains it nothing), with the launch shapes tuned for a lone instance:
+    # all results are the same bit for bit, also the strict schedule's
     g_alone = insts[0].grad(A[0])
-    ish = W.make_instance(masks[0], T, K_amp, precision=fkadj.F32, ckpt=ckpt, fwd_tile=fwd_tile, adj_tile=adj_tile)
+    from fkadj import autotune
+    ish = W.make_instance(masks[0], T, K_amp, precision=fkadj.F32, ckpt=ckpt,
+                          **autotune.instance_kwargs(N, batched=False))
     ish.set_state(states[0])
     lat = []
     for k in range(args.latency_evals):

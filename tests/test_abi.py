@@ -165,6 +165,22 @@ def test_plan_and_server_argument_errors_before_device_work():
     assert lib.fkadj_server_pending(None) == L.ERR_ARG
 
 
+def test_launch_shape_fields_checked_before_device_work():
+    """fwd_depth and adj_depth outside 0..2 and roll_rows that is not a multiple of 8 in 0..256 are
+    refused with FKADJ_ERR_ARG before any device work (and before the geometry is read)."""
+    import ctypes as C
+    ai = np.ones(4, np.int32)   # never read: the argument checks come first
+    for field, bad in (("fwd_depth", 3), ("fwd_depth", -1), ("adj_depth", 3), ("adj_depth", -2),
+                       ("roll_rows", 12), ("roll_rows", 264), ("roll_rows", -8)):
+        p = L.Problem()
+        p.nrows, p.ncols, p.nt, p.dt, p.precision = 64, 64, 10, 0.1, L.F32
+        p.pulse_mode = L.PULSE_KNOTS
+        p.active_index = ai.ctypes.data_as(C.POINTER(C.c_int32))
+        setattr(p, field, bad)
+        h = C.c_void_p()
+        assert fkadj.lib().fkadj_create(C.byref(p), C.byref(h)) == L.ERR_ARG, (field, bad)
+
+
 def test_grid_too_large_for_32_bit_offsets():
     p = L.Problem()
     p.nrows, p.ncols, p.nt, p.dt, p.precision = 15000, 15000, 10, 0.1, L.F32

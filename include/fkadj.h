@@ -273,7 +273,9 @@ int fkadj_set_rolling(int mode, int strip_rows);
 /* ---------------------------------------------------------------- serving */
 /* Continuous-batching gradient server on one GPU (one host thread drives it).  A request is
  * admitted when a slot is free and its evaluation buffers (checkpoints, segment states, slopes,
- * adjoint states, tables: fkadj_request_bytes of its instance) fit the remaining budget; the buffers are allocated at admission and freed when the request finishes.  Admitted
+ * adjoint states, tables: fkadj_request_bytes of its instance) fit the remaining budget; a finished
+ * request's buffers are kept for the next request of the same shape (counted against the budget,
+ * freed when a request of another shape needs the room).  Admitted
  * requests advance together, one forward-kind and one adjoint launch per round over all slots,
  * each slot at its own step and phase; new requests join and finished ones leave at round
  * boundaries.  Each result equals the same request evaluated alone (fkadj_grad on the instance)

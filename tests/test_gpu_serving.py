@@ -262,3 +262,31 @@ def test_nonfinite_request_fails_only_itself():
         assert np.array_equal(alone[1][0], r[0])
     for inst, _ in pairs:
         inst.close()
+
+
+@pytest.mark.parametrize("roll_mode", [0, 2])
+def test_slot_reuse_and_rolling_rounds_equal_alone(roll_mode):
+    """More requests than slots over several pitches, so that freed slots pass to later requests, and
+    (mode 2) the pitches the row-rolling kernels take run on them in the server's rounds.  Every result
+    equals its instance alone (evaluated on the tiled kernels)."""
+    _print_case("server_slot_reuse", roll_mode=roll_mode)
+    specs = [(48, 256, 60, 6, 20, 11), (40, 384, 50, 5, 25, 12), (32, 200, 40, 4, 10, 13), (56, 256, 45, 5, 15, 14),
+             (24, 512, 30, 3, 10, 15), (40, 320, 36, 4, 12, 16), (36, 384, 44, 4, 11, 17)]
+    pairs = _instances(specs)
+    prev = fkadj.set_rolling(0, 16)
+    try:
+        alone = [inst.grad(A) for inst, A in pairs]
+        fkadj.set_rolling(roll_mode, 16)
+        with fkadj.Server(budget_bytes=1 << 34, max_slots=3) as srv:
+            ids = [srv.submit(inst, A) for inst, A in pairs[:4]]
+            srv.run(7)
+            ids += [srv.submit(inst, A) for inst, A in pairs[4:]]
+            while srv.pending():
+                srv.run(0)
+            res = [srv.poll(i) for i in ids]
+    finally:
+        fkadj.set_rolling(prev, 16)
+    for a, r in zip(alone, res):
+        assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
+    for inst, _ in pairs:
+        inst.close()

@@ -131,3 +131,29 @@ def test_mixed_pitch_batch_rolls_per_pitch():
     _same(alone, mixed)
     for i in insts:
         i.close()
+
+
+@pytest.mark.parametrize("W,H,hs", [(256, 4, 8), (384, 5, 16), (512, 9, 8), (256, 17, 16), (384, 7, 8)])
+def test_rolling_equals_tiled_short_grids(W, H, hs):
+    """Grids of 4 to 17 rows: one strip shorter than its strip height and than the four skewed stages,
+    and a last strip of one row that rolls upwards (odd strip index); single-cell holes next to the
+    strip boundaries.  Rolling equals tiled bit for bit."""
+    _print_case("roll_short", W=W, H=H, hs=hs)
+    mask = np.ones((H, W - 3), bool)
+    for r, c in ((H // 2, 40), (min(hs, H - 2), 90), (1, W // 2), (H - 2, W - 20)):
+        mask[r, c] = False
+    mask = G.fill_isthmuses(mask)   # no active cell without a neighbour on an axis
+    ai = G.active_index_from_mask(mask)
+    n = int(mask.sum())
+    rng = np.random.default_rng(H * 7 + W)
+    y = np.concatenate([rng.uniform(0, 1, n), rng.uniform(0.5, 1, n), rng.uniform(0.5, 1, n)])
+    inst = fkadj.Instance(ai, H, W - 3, dt=0.02, nt=30, model=fkadj.reference_model(h=0.025),
+                          pulse_mode=fkadj.PULSE_PIECEWISE, n_amp=3, alpha=2e-3, gamma=(1, 0.2, 0.2),
+                          S=G.S_matrix(mask), precision=fkadj.F32, ckpt=10)
+    inst.set_state(y)
+    E = rng.uniform(0, 10, 3)
+    tiled = _grad(0, hs, [inst], [E])
+    rolled = _grad(2, hs, [inst], [E])
+    assert np.all(np.isfinite(tiled[0][0]))
+    _same(tiled, rolled)
+    inst.close()

@@ -409,22 +409,26 @@ def run_c3(args):
         log(f"[c3] stream launches, step {k + 1}/{args.steps}: {stream_ms[-1]:.0f} ms")
     tm = fkadj.last_timing()
     fkadj.set_profiling(False)
-    ts = time.perf_counter()
-    plan = fkadj.Plan(insts)
-    capture_s = time.perf_counter() - ts
-    log(f"[c3] graph captured and instantiated in {capture_s:.1f} s")
-    for w in range(args.warmup):
-        plan.grad(A)
-    step_ms = []
-    t_start = time.perf_counter()
-    for k in range(args.steps):
+    if args.c3_no_plan:
+        # direct stream launches only (the value is theirs): the profiler's kernel trace of this leg
+        capture_s, step_ms, total, res, graph_same = None, stream_ms, 1e-3 * sum(stream_ms), res_stream, None
+    else:
         ts = time.perf_counter()
-        res = plan.grad(A)
-        step_ms.append(1e3 * (time.perf_counter() - ts))
-        log(f"[c3] graph replay, step {k + 1}/{args.steps}: {step_ms[-1]:.0f} ms")
-    total = time.perf_counter() - t_start
-    plan.close()
-    graph_same = all(np.array_equal(a[0], b[0]) and a[1] == b[1] for a, b in zip(res, res_stream))
+        plan = fkadj.Plan(insts)
+        capture_s = time.perf_counter() - ts
+        log(f"[c3] graph captured and instantiated in {capture_s:.1f} s")
+        for w in range(args.warmup):
+            plan.grad(A)
+        step_ms = []
+        t_start = time.perf_counter()
+        for k in range(args.steps):
+            ts = time.perf_counter()
+            res = plan.grad(A)
+            step_ms.append(1e3 * (time.perf_counter() - ts))
+            log(f"[c3] graph replay, step {k + 1}/{args.steps}: {step_ms[-1]:.0f} ms")
+        total = time.perf_counter() - t_start
+        plan.close()
+        graph_same = all(np.array_equal(a[0], b[0]) and a[1] == b[1] for a, b in zip(res, res_stream))
     g_alone = insts[k0].grad(A[k0])
     same = bool(np.array_equal(g_alone[0], res[k0][0]))
     inst64 = W.make_instance(masks[k0], T, K_amp, precision=fkadj.F64, ckpt=50)
@@ -439,7 +443,8 @@ def run_c3(args):
                                   f"{[sizes.count(n) for n in sorted(set(sizes))]}), T={T:g} ms, {K_amp} amplitudes, K=50",
                       "cells": cells, "nt": nt, "kept_segments": kept},
            "alone_equals_batched": same, "graph_equals_stream": graph_same,
-           "launch": "captured HIP graph of the whole evaluation (replayed per step)",
+           "launch": ("direct stream launches" if args.c3_no_plan
+                      else "captured HIP graph of the whole evaluation (replayed per step)"),
            "graph_capture_s": capture_s,
            "stream_launches": {"value": B * args.steps / (1e-3 * sum(stream_ms)), "unit": "evals/s",
                                "ms_per_step": float(np.mean(stream_ms))},
@@ -542,6 +547,8 @@ def main():
     ap.add_argument("--workload", choices=("c1", "c2", "c3", "c4", "c5"), default="c2",
                     help="c2 (default): 16 x C2 batched; c1: 128^2 FP64 with the CPU oracle; c3: 64 uneven "
                          "grids; c4: served Poisson stream; c5: 2048^2, 25 000 steps")
+    ap.add_argument("--c3-no-plan", action="store_true",
+                    help="c3: time the direct stream launches only (no graph capture)")
     ap.add_argument("--c4-instances", type=int, default=24)
     ap.add_argument("--c4-requests", type=int, default=96)
     ap.add_argument("--c4-rate", type=float, default=200.0, help="offered load of the C4 stream (req/s)")

@@ -236,7 +236,7 @@ template <typename T>
 T *as(char *p) { return reinterpret_cast<T *>(p); }
 
 void group_free(FusedGroupDev *g);
-int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out);
+int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out, int concurrent = 1);
 
 void fill_models(fkadj_instance *I)
 {
@@ -722,12 +722,51 @@ int adj_cfg_of(const fkadj_instance *I, int64_t launch_cells)
 // Row-rolling kernels: 0 never, 1 for groups of 2M cells or more (default), 2 for every group that
 // can use them; strip rows (a multiple of 8 and of every member's adjoint tile rows).
 std::atomic<int> g_roll_mode{1};
-std::atomic<int> g_roll_hs{16};
+std::atomic<int> g_roll_hs{0};   // strip rows of the rolling launches; 0: chosen per launch (auto_strip_rows)
 constexpr int64_t kRollCells = (int64_t)2 << 20;
 
+// Compute units of the current device (queried once per device).
+int device_cus()
+{
+    static std::mutex mu;
+    static std::map<int, int> cus;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    std::lock_guard<std::mutex> lk(mu);
+    auto it = cus.find(dev);
+    if (it != cus.end()) return it->second;
+    int n = 0;
+    if (hipDeviceGetAttribute(&n, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || n < 1) n = 256;
+    cus[dev] = n;
+    return n;
+}
+
+// Strip rows of a rolling launch over instances of these heights at this pitch, when neither the
+// members nor fkadj_set_rolling fix them: the tallest of 64, 32 and 16 rows whose strips still give
+// the launch its share of the workgroups the GPU holds at once (CUs x workgroups of this width per
+// CU, at the kernels' 16 waves per CU, divided by the launches that run side by side), else 16.
+// Taller strips evaluate fewer rows twice at the strip ends (6 per strip) and re-read fewer; a
+// launch with fewer strips than its share leaves CUs idle.  Sixteen 512^2 instances in two groups
+// get 16 rows; a group of ten 1024^2 instances gets 64 and one of ten 768^2 instances 32 (64 uneven
+// grids of 256^2..1024^2, T = 150 ms: 8.06 -> 7.41 s per batched evaluation).  Results do not
+// depend on it.
+int auto_strip_rows(const std::vector<int> &rows, int pitch, int concurrent)
+{
+    const int per_cu = std::max(1, 16 / (pitch / 64));
+    const int64_t share = (int64_t)device_cus() * per_cu / std::max(1, concurrent);
+    for (int hs : {64, 32, 16}) {
+        int64_t strips = 0;
+        for (int r : rows) strips += (r + hs - 1) / hs;
+        if (strips >= share) return hs;
+    }
+    return 16;
+}
+
 // Host part of a group: members, launch maps, depth and kept-segment choice; the slot records go to
-// `slots` in member order (no device work).
-void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::vector<FusedSlot> &slots)
+// `slots` in member order (no device work).  `concurrent`: groups of the same call that run side by
+// side (the share of the GPU a rolling launch should fill).
+void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::vector<FusedSlot> &slots,
+                int concurrent = 1)
 {
     g->members = members;
     slots.clear();
@@ -767,14 +806,16 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         if (I->adj_depth != I0->adj_depth) g->adj_depth = 1;
     }
     // row-rolling kernels per pitch they support: every such pitch (mode 2) or those with 2M cells
-    // or more in the group (mode 1); strips of hs rows (a multiple of the partial block height)
-    // (the members' own strip rows, fkadj_problem.roll_rows, where all members of the pitch ask for
-    // the same, else the library setting)
+    // or more in the group (mode 1); strips of hs rows (a multiple of the partial block height):
+    // the members' own strip rows (fkadj_problem.roll_rows) where all members of the pitch ask for
+    // the same, else the library setting, else (setting 0) auto_strip_rows
     const int rmode = g_roll_mode.load(), hs = g_roll_hs.load();
     std::map<int, int64_t> pitch_cells;
     std::map<int, int> pitch_rows;   // -1: the members disagree
+    std::map<int, std::vector<int>> pitch_heights;
     for (auto *C : members) {
         pitch_cells[C->I->pitch] += (int64_t)C->I->rows * C->I->pitch;
+        pitch_heights[C->I->pitch].push_back(C->I->rows);
         auto it = pitch_rows.find(C->I->pitch);
         if (it == pitch_rows.end()) pitch_rows[C->I->pitch] = C->I->roll_rows;
         else if (it->second != C->I->roll_rows) it->second = -1;
@@ -784,7 +825,9 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
         if (roll_pitch_supported(kv.first) && (rmode == 2 || (rmode == 1 && kv.second >= kRollCells)) &&
             g->nroll < FusedGroupDev::kMaxRoll) {
             g->roll_pitch[g->nroll] = kv.first;
-            g->roll_hs[g->nroll] = pitch_rows[kv.first] > 0 ? pitch_rows[kv.first] : hs;
+            g->roll_hs[g->nroll] = pitch_rows[kv.first] > 0 ? pitch_rows[kv.first]
+                                   : hs > 0                 ? hs
+                                                            : auto_strip_rows(pitch_heights[kv.first], kv.first, concurrent);
             launch_map_clear(g->rollmap[g->nroll]);
             g->nroll++;
         }
@@ -808,13 +851,13 @@ void group_maps(FusedGroupDev *g, const std::vector<EvalCtx *> &members, std::ve
     if (g->nroll) g->adj_depth = 1;
 }
 
-int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out)
+int group_build(const std::vector<EvalCtx *> &members, hipStream_t s, FusedGroupDev **out, int concurrent)
 {
     if (members.empty() || members.size() > (size_t)MAX_SLOTS) return FKADJ_ERR_ARG;
     auto *g = new (std::nothrow) FusedGroupDev();
     if (!g) return FKADJ_ERR_NOMEM;
     std::vector<FusedSlot> slots;
-    group_maps(g, members, slots);
+    group_maps(g, members, slots, concurrent);
     bool ok = hipMalloc((void **)&g->d_slots, sizeof(FusedSlot) * slots.size()) == hipSuccess &&
               hipMemcpyAsync(g->d_slots, slots.data(), sizeof(FusedSlot) * slots.size(), hipMemcpyHostToDevice, s) == hipSuccess;
     ok = ok && hipStreamCreateWithFlags(&g->side, hipStreamNonBlocking) == hipSuccess &&
@@ -883,7 +926,7 @@ int groups_build(const std::vector<EvalCtx *> &members_in, hipStream_t s, int ma
         for (; q < end[j]; q++) parts[j].push_back(members[q]);
     for (auto &part : parts) {
         FusedGroupDev *g = nullptr;
-        const int rc = group_build(part, s, &g);
+        const int rc = group_build(part, s, &g, k);
         if (rc) { groups_free(out); return rc; }
         out.push_back(g);
     }
@@ -1591,7 +1634,7 @@ int fkadj_set_batch_groups(int n)
 
 int fkadj_set_rolling(int mode, int strip_rows)
 {
-    if (mode < 0 || mode > 2 || strip_rows < 8 || strip_rows > 256 || strip_rows % 8 != 0) return FKADJ_ERR_ARG;
+    if (mode < 0 || mode > 2 || strip_rows < 0 || strip_rows > 256 || strip_rows % 8 != 0) return FKADJ_ERR_ARG;
     g_roll_hs.store(strip_rows);
     return g_roll_mode.exchange(mode);
 }
@@ -2247,6 +2290,7 @@ int server_round(fkadj_server *S)
     for (int k = 0; k < MAX_SLOTS; k++) fc.c[k] = ac.c[k] = rc_rhs.c[k] = SlotCursor{MODE_IDLE, 0, 0, 0};
     int64_t fcells = 0, acells = 0;   // cells of this round's forward-kind and adjoint launches
     std::map<int, int64_t> fpc, apc;  // the same per pitch
+    std::map<int, std::vector<int>> fph, aph;   // heights of the instances per pitch
     for (int k = 0; k < S->max_slots; k++)
         if (S->slot_req[k]) {
             const fkadj_instance *I = S->slot_req[k]->I;
@@ -2254,6 +2298,7 @@ int server_round(fkadj_server *S)
             const bool fw = S->slot_req[k]->phase != PH_ADJ;
             (fw ? fcells : acells) += cells;
             (fw ? fpc : apc)[I->pitch] += cells;
+            (fw ? fph : aph)[I->pitch].push_back(I->rows);
         }
     // the slots of a pitch that the row-rolling kernels support take them when that pitch alone has
     // 4M cells or more in the launch (one launch per such pitch; the same results as the tiled
@@ -2265,6 +2310,11 @@ int server_round(fkadj_server *S)
         if (!roll_pitch_supported(pitch)) return false;
         return rmode == 2 || (rmode == 1 && pc.at(pitch) >= 2 * kRollCells);
     };
+    // strip rows per pitch and launch kind: the library setting, else auto_strip_rows (one launch at
+    // a time on the server's stream)
+    std::map<int, int> fhs, ahs;
+    for (auto &kv : fph) fhs[kv.first] = hs > 0 ? hs : auto_strip_rows(kv.second, kv.first, 1);
+    for (auto &kv : aph) ahs[kv.first] = hs > 0 ? hs : auto_strip_rows(kv.second, kv.first, 1);
     std::map<int, LaunchMap> rfm, ram;
     auto rmap = [](std::map<int, LaunchMap> &mm, int pitch) -> LaunchMap & {
         auto it = mm.find(pitch);
@@ -2283,22 +2333,22 @@ int server_round(fkadj_server *S)
             ac.c[k] = SlotCursor{MODE_ADJ, r->step, r->seg * K, r->zpar};
             const int ac = adj_cfg_of(I, acells);
             if (roll_for(apc, I->pitch))
-                launch_map_add(rmap(ram, I->pitch), k, (I->rows + hs - 1) / hs);
+                launch_map_add(rmap(ram, I->pitch), k, (I->rows + ahs[I->pitch] - 1) / ahs[I->pitch]);
             else
                 launch_map_add(am[ac], k, tiles_of(I->rows, I->pitch, ac));
         } else {
             fc.c[k] = r->phase == PH_FWD ? SlotCursor{MODE_FWD, r->step, 0, 0}
                                          : SlotCursor{MODE_REC, r->seg * K + r->step, r->seg * K, 0};
             if (roll_for(fpc, I->pitch))
-                launch_map_add(rmap(rfm, I->pitch), k, (I->rows + hs - 1) / hs);
+                launch_map_add(rmap(rfm, I->pitch), k, (I->rows + fhs[I->pitch] - 1) / fhs[I->pitch]);
             else
                 launch_map_add(fm[f], k, tiles_of(I->rows, I->pitch, f));
         }
     }
     for (int c = 1; c < NUM_TILE_CFG; c++) fused_forward_launch(c, S->d_slots, fm[c], fc, S->stream);
-    for (auto &kv : rfm) roll_launch(false, kv.first, S->d_slots, kv.second, fc, hs, S->stream);
+    for (auto &kv : rfm) roll_launch(false, kv.first, S->d_slots, kv.second, fc, fhs[kv.first], S->stream);
     for (int c = 1; c < NUM_TILE_CFG; c++) fused_adjoint_launch(c, S->d_slots, am[c], ac, S->stream);
-    for (auto &kv : ram) roll_launch(true, kv.first, S->d_slots, kv.second, ac, hs, S->stream);
+    for (auto &kv : ram) roll_launch(true, kv.first, S->d_slots, kv.second, ac, ahs[kv.first], S->stream);
     HIPCHK(hipGetLastError());
     for (int k = 0; k < S->max_slots; k++) {
         ServReq *r = S->slot_req[k];

@@ -163,9 +163,10 @@ def set_batch_groups(n: int) -> int:
     return rc
 
 
-def set_rolling(mode: int, strip_rows: int = 16) -> int:
+def set_rolling(mode: int, strip_rows: int = 0) -> int:
     """Row-rolling kernels for large one-pitch FP32 groups: 0 never, 1 from 2M cells (default), 2
-    whenever possible; strip_rows a multiple of 8.  Returns the previous mode."""
+    whenever possible; strip_rows a multiple of 8, or 0 (default) for strip rows chosen per launch.
+    Returns the previous mode."""
     rc = lib().fkadj_set_rolling(int(mode), int(strip_rows))
     check(min(rc, 0), "fkadj_set_rolling")
     return rc

@@ -11,8 +11,10 @@ fastest in tuned_tiles.json, at two launch scales:
   and depth (64x8/512 with one or two steps per launch, 64x16/512 with one), the adjoint swept with
   the best forward choice.
 * "batch": instances batched to about 4M cells per launch.  Candidates: the row-rolling kernels
-  with strips of 8, 16 or 32 rows (pitches 256, 384, 512, 768, 1024), and the tiled kernels
-  (64x16/512) for every pitch.
+  (pitches 256, 384, 512, 768, 1024) with strip rows chosen per launch by the library or fixed at 8,
+  16 or 32, and the tiled kernels (64x16/512) for every pitch.  A fixed strip height is stored only
+  where it beats the library's per-launch choice by more than 5% (the per-launch choice also follows
+  the batch's size, which a fixed height measured on one batch does not).
 
 Tile ids: 1 = 64x16/256, 2 = 64x32/256, 3 = 64x32/512, 4 = 64x16/512, 5 = 64x32/1024,
 6 = 64x8/512, 7 = 64x8/256.
@@ -43,7 +45,7 @@ ROLL_PITCHES = (256, 384, 512, 768, 1024)
 # single-instance candidates: (fwd_tile, fwd_depth) and (adj_tile, adj_depth)
 FWD_SINGLE = ((6, 2), (6, 1), (4, 1))
 ADJ_SINGLE = ((6, 1), (4, 1), (6, 2))
-# batched candidates: strip rows of the rolling kernels (0: tiled kernels only)
+# batched candidates: fixed strip rows of the rolling kernels (besides the library's per-launch choice)
 ROLL_ROWS = (8, 16, 32)
 
 
@@ -65,7 +67,7 @@ def synthetic_batch(N: int, B: int, T: float, holes: int | None = None, **kw):
 def time_config(N: int, B: int, T: float, reps: int = 2, roll_mode: int = 1, **kw) -> dict:
     """Best of `reps` wall-clock batched evaluations (after one warm-up) under the given Instance
     keywords; the rolling kernels are allowed (roll_mode 1, the library default) or not (0)."""
-    prev = L.set_rolling(roll_mode, 16)
+    prev = L.set_rolling(roll_mode, 0)
     insts, A = synthetic_batch(N, B, T, **kw)
     try:
         batch_grad(insts, A)
@@ -79,7 +81,7 @@ def time_config(N: int, B: int, T: float, reps: int = 2, roll_mode: int = 1, **k
     finally:
         for i in insts:
             i.close()
-        L.set_rolling(prev, 16)
+        L.set_rolling(prev, 0)
 
 
 def sweep_single(N: int, T: float) -> dict:
@@ -104,21 +106,26 @@ def sweep_single(N: int, T: float) -> dict:
 
 
 def sweep_batch(N: int, T: float, cells: int = 1 << 22) -> dict:
-    """Rolling strip rows (pitches the rolling kernels take) against the tiled kernels."""
+    """The rolling kernels (strip rows chosen per launch, or fixed) against the tiled kernels."""
     B = min(64, max(1, round(cells / (N * N))))
     pitch = (N + 63) // 64 * 64
-    res = {0: time_config(N, B, T, roll_mode=0, fwd_tile=4, adj_tile=4)}
-    print(f"{N}^2 x{B} tiled 64x16/512: {res[0]['ms']:.1f} ms/eval", flush=True)
+    res = {"tiled": time_config(N, B, T, roll_mode=0, fwd_tile=4, adj_tile=4)}
+    print(f"{N}^2 x{B} tiled 64x16/512: {res['tiled']['ms']:.1f} ms/eval", flush=True)
     if pitch in ROLL_PITCHES:
+        res["auto"] = time_config(N, B, T, roll_mode=2, fwd_tile=4, adj_tile=4)
+        print(f"{N}^2 x{B} rolling, strip rows chosen per launch: {res['auto']['ms']:.1f} ms/eval", flush=True)
         for hs in ROLL_ROWS:
             res[hs] = time_config(N, B, T, roll_mode=2, fwd_tile=4, adj_tile=4, roll_rows=hs)
             print(f"{N}^2 x{B} rolling strips of {hs} rows: {res[hs]['ms']:.1f} ms/eval", flush=True)
     best = min(res, key=lambda c: res[c]["ms"])
-    same = all(np.array_equal(r["grad0"], res[0]["grad0"]) for r in res.values())
-    print(f"{N}^2 x{B} best: {'tiled' if best == 0 else f'rolling, {best}-row strips'}; same gradient: {same}",
-          flush=True)
-    return {"batch": B, "roll_rows": best, "fwd_tile": 4, "adj_tile": 4, "ms": res[best]["ms"],
-            "candidates_ms": {("tiled" if k == 0 else f"roll {k}"): v["ms"] for k, v in res.items()},
+    if isinstance(best, int) and res[best]["ms"] > 0.95 * res["auto"]["ms"]:
+        best = "auto"   # a fixed height only where it clearly beats the per-launch choice
+    same = all(np.array_equal(r["grad0"], res["tiled"]["grad0"]) for r in res.values())
+    desc = {"tiled": "tiled", "auto": "rolling, strip rows chosen per launch"}.get(best, f"rolling, {best}-row strips")
+    print(f"{N}^2 x{B} best: {desc}; same gradient: {same}", flush=True)
+    return {"batch": B, "choice": "tiled" if best == "tiled" else "roll",
+            "roll_rows": best if isinstance(best, int) else 0, "fwd_tile": 4, "adj_tile": 4, "ms": res[best]["ms"],
+            "candidates_ms": {(k if isinstance(k, str) else f"roll {k}"): v["ms"] for k, v in res.items()},
             "same_gradient": same}
 
 
@@ -139,6 +146,8 @@ def instance_kwargs(N: int, batched: bool, path: str = TABLE) -> dict:
     if batched:
         if e["roll_rows"] > 0:
             return {"roll_rows": int(e["roll_rows"])}
+        if e.get("choice") == "roll":
+            return {}   # rolling kernels with strip rows chosen per launch
         return {"fwd_tile": int(e["fwd_tile"]), "adj_tile": int(e["adj_tile"])}
     return {k: int(e[k]) for k in ("fwd_tile", "fwd_depth", "adj_tile", "adj_depth")}
 

@@ -266,7 +266,9 @@ int fkadj_set_batch_groups(int n);
  * such pitch, the other members in the tiled launches of the same step.  Results equal the tiled
  * kernels' bit for bit.  mode 0: never; 1 (default): the pitches with 2M cells or more in the group,
  * and in a server round the pitches with 4M cells or more; 2: every such pitch.  strip_rows: a
- * multiple of 8 (8..256; default 16).  Applies to groups built after the call (plans keep their
+ * multiple of 8 (8..256), or 0 (default): chosen per launch, the tallest of 64, 32 and 16 rows whose
+ * strips still give the launch its share of the workgroups the GPU holds at once (the groups of a
+ * split batch share it), else 16.  Applies to groups built after the call (plans keep their
  * kernels).  Process-wide; returns the previous mode.  No reference counterpart. */
 int fkadj_set_rolling(int mode, int strip_rows);
 

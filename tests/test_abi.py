@@ -201,13 +201,17 @@ def test_tuned_table_lookup(tmp_path, monkeypatch):
                 "batch": {"roll_rows": 16, "fwd_tile": 4, "adj_tile": 4, "batch": 64, "ms": 2.0}},
         "1024": {"single": {"fwd_tile": 4, "fwd_depth": 1, "adj_tile": 4, "adj_depth": 1, "ms": 3.0},
                  "batch": {"roll_rows": 0, "fwd_tile": 4, "adj_tile": 4, "batch": 4, "ms": 4.0}},
-        "2048": {"single": {"fwd_tile": 4, "fwd_depth": 1, "adj_tile": 6, "adj_depth": 2, "ms": 5.0}}}))
+        "2048": {"single": {"fwd_tile": 4, "fwd_depth": 1, "adj_tile": 6, "adj_depth": 2, "ms": 5.0}},
+        "512": {"batch": {"choice": "roll", "roll_rows": 0, "fwd_tile": 4, "adj_tile": 4, "batch": 16, "ms": 6.0}},
+        "768": {"batch": {"choice": "tiled", "roll_rows": 0, "fwd_tile": 4, "adj_tile": 4, "batch": 7, "ms": 7.0}}}))
     assert autotune.instance_kwargs(300, False, str(t)) == {"fwd_tile": 6, "fwd_depth": 2, "adj_tile": 6,
                                                             "adj_depth": 1}
     assert autotune.instance_kwargs(256, True, str(t)) == {"roll_rows": 16}
     assert autotune.instance_kwargs(900, True, str(t)) == {"fwd_tile": 4, "adj_tile": 4}
     assert autotune.instance_kwargs(4096, True, str(t)) == {"fwd_tile": 4, "adj_tile": 4}   # nearest batch entry
     assert autotune.instance_kwargs(2048, False, str(t))["adj_depth"] == 2
+    assert autotune.instance_kwargs(512, True, str(t)) == {}   # rolling, strip rows chosen per launch
+    assert autotune.instance_kwargs(760, True, str(t)) == {"fwd_tile": 4, "adj_tile": 4}
     monkeypatch.setattr(autotune, "load_table", lambda path=None: {})
     assert autotune.instance_kwargs(512, True) == {} and autotune.instance_kwargs(512, False) == {}
 
@@ -296,15 +300,16 @@ def test_batch_calls_check_field_lengths():
 
 
 def test_rolling_setting():
-    """The row-rolling setting takes modes 0..2 and strip heights that are multiples of 8 (no device
-    work) and returns the previous mode."""
+    """The row-rolling setting takes modes 0..2 and strip heights that are multiples of 8, 0 meaning
+    chosen per launch (no device work), and returns the previous mode."""
     prev = fkadj.set_rolling(2, 32)
     try:
         assert prev in (0, 1, 2)
         assert fkadj.set_rolling(0, 8) == 2
-        for mode, hs in ((3, 16), (-1, 16), (1, 12), (1, 0), (1, 264)):
+        assert fkadj.set_rolling(1, 0) == 0
+        for mode, hs in ((3, 16), (-1, 16), (1, 12), (1, -8), (1, 264)):
             with pytest.raises(fkadj.FkadjError) as e:
                 fkadj.set_rolling(mode, hs)
             assert e.value.code == L.ERR_ARG
     finally:
-        fkadj.set_rolling(prev, 16)
+        fkadj.set_rolling(prev, 0)

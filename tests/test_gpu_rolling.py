@@ -20,7 +20,7 @@ def _print_case(name, **kw):
 def _restore_rolling():
     prev = fkadj.set_rolling(1, 16)
     yield
-    fkadj.set_rolling(prev, 16)
+    fkadj.set_rolling(prev, 0)
 
 
 def _front_state(case):

@@ -285,7 +285,7 @@ def test_slot_reuse_and_rolling_rounds_equal_alone(roll_mode):
                 srv.run(0)
             res = [srv.poll(i) for i in ids]
     finally:
-        fkadj.set_rolling(prev, 16)
+        fkadj.set_rolling(prev, 0)
     for a, r in zip(alone, res):
         assert np.array_equal(a[0], r[0]) and a[1] == r[1] and a[2] == r[2]
     for inst, _ in pairs:

@@ -54,10 +54,10 @@ def _same(a, b):
 
 
 @pytest.mark.parametrize("W,H,hs,adj_tile", [(512, 70, 16, 0), (256, 64, 8, 0), (384, 45, 32, 4),
-                                             (256, 33, 16, 6), (512, 40, 8, 6)])
+                                             (256, 33, 16, 6), (512, 40, 8, 6), (1024, 130, 64, 0)])
 def test_rolling_equals_tiled_alone(W, H, hs, adj_tile):
-    """Widths 256..512 (ragged column count inside the pitch too), rows not a multiple of the strip,
-    strips of 8..32 rows, both adjoint tiles; recompute segments of 20 steps over 60 steps."""
+    """Widths 256..1024 (ragged column count inside the pitch too), rows not a multiple of the strip,
+    strips of 8..64 rows, both adjoint tiles; recompute segments of 20 steps over 60 steps."""
     _print_case("roll_alone", W=W, H=H, hs=hs, adj_tile=adj_tile)
     inst, E = _instance(H, W - 7, 60, 6, 20, 3 + H, adj_tile=adj_tile)
     tiled = _grad(0, hs, [inst], [E])
